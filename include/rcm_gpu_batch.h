@@ -1,0 +1,129 @@
+/*
+ * rcm_gpu_batch.h -- C ABI of the MI355X (gfx950) Cuthill-McKee engine.
+ *
+ * Drop-in for the reference's GPU_BATCH path. Every entry point replaces a reference interface, cited as
+ * file:line in the reference repository (GPUPeople/ParallelBatchRCM):
+ *   rcm_gpu_batch                 GPU_BATCH::rcm<T>             include/rcm_gpu_batch.h:40-41,
+ *                                                              src/device/rcm_gpu_batch.cu:42-112
+ *   rcm_ctx_find_start            SimplePeripheral::findPeripheral include/simple_peripheral.h:45-46,
+ *                                                              src/simple_peripheral.cpp:92-141
+ *   rcm_ctx_bandwidth             computeBandwidth              src/main.cpp:143-184
+ *   rcm_ctx_run / rcm_ctx_load    the device half of GPU_BATCH::rcm (upload, init kernel, persistent run
+ *                                 kernel, download; src/device/rcm_gpu_batch.cu:57-102) split so callers
+ *                                 can keep the CSR resident in HBM between calls
+ *   rcm_gpu_batch_many            no single reference counterpart: GPU_BATCH::rcm applied to many matrices,
+ *                                 done in one persistent launch
+ *
+ * Conventions
+ *   - CSR is int32: row_ptr[n+1], col_idx[nnz]; 1 <= n < 2^30, nnz < 2^31. Values are not needed.
+ *   - Output perm[new] = old. Without RCM_FLAG_REVERSE it is Cuthill-McKee order (what the reference's
+ *     GPU_BATCH::rcm and RealCPU::rcm return); with it, the reverse (RCM).
+ *   - The order is the stable one: bit-identical to RealCPU::rcm_seq<true> (src/rcm_cpu.cpp:57-104) from the
+ *     same start node. RCM_FLAG_STABLE is accepted for interface parity; the engine is always stable.
+ *   - No exceptions cross this boundary; nothing calls exit()/abort(). Functions return RCM_OK (0) or a
+ *     negative rcm_status. A disconnected graph returns RCM_ENOTCONNECTED (the reference kernel hangs).
+ *   - Thread safety: one rcm_ctx per host thread; distinct contexts are independent.
+ */
+#ifndef RCM_GPU_BATCH_H
+#define RCM_GPU_BATCH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rcm_flags {
+  RCM_FLAG_REVERSE = 1u << 0,   /* return RCM (reversed CM) instead of CM */
+  RCM_FLAG_STABLE = 1u << 1,    /* stable degree sort (always on) */
+  RCM_FLAG_BANDWIDTH = 1u << 2  /* also compute the bandwidth of the reordered matrix */
+};
+
+enum rcm_status {
+  RCM_OK = 0,
+  RCM_EINVAL = -1,         /* malformed CSR, start node out of range, bad argument */
+  RCM_ENOTCONNECTED = -2,  /* the start's component does not cover all n nodes */
+  RCM_ENOMEM = -3,         /* device or host allocation failed */
+  RCM_EDEVICE = -4,        /* HIP runtime error */
+  RCM_ETIMEOUT = -5,       /* device watchdog expired (see RCM_PARAM_TIMEOUT_MS) */
+  RCM_ECAPACITY = -6,      /* an internal capacity was exceeded */
+  RCM_ENOTLOADED = -7      /* context has no matrix loaded */
+};
+
+/* One-shot call on host buffers: upload, (optional GPU start search), reorder, (optional bandwidth),
+ * download. start_node < 0 selects the pseudo-peripheral search (== SimplePeripheral::findPeripheral).
+ * perm_out: caller-allocated n entries. bandwidth_out, start_node_out: nullable. */
+int rcm_gpu_batch(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int32_t start_node,
+                  uint32_t flags, int32_t* perm_out, int64_t* bandwidth_out, int32_t* start_node_out);
+
+/* ---- device-resident context ---- */
+typedef struct rcm_ctx rcm_ctx;
+
+enum rcm_param {
+  RCM_PARAM_WORKERS = 1,     /* persistent workgroups for one matrix (0 = autotuned) */
+  RCM_PARAM_TIMEOUT_MS = 2,  /* device watchdog, default 20000 */
+  RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
+  RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width: 2, 4 (default) or 6 */
+  RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads/2048 slots, 1 = 256/4096, 2 = 512/4096 */
+  RCM_PARAM_STRICT = 6       /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
+};
+
+int rcm_ctx_create(int device, rcm_ctx** out);
+int rcm_ctx_destroy(rcm_ctx* ctx);
+int rcm_ctx_set_param(rcm_ctx* ctx, int param, int64_t value);
+int rcm_ctx_get_param(rcm_ctx* ctx, int param, int64_t* value);
+
+/* Copy a host CSR into the context (validated on the device). */
+int rcm_ctx_load(rcm_ctx* ctx, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx);
+/* Use a CSR already in device memory (not copied; must stay valid while the context uses it). */
+int rcm_ctx_load_device(rcm_ctx* ctx, int32_t n, int64_t nnz, const int32_t* d_row_ptr, const int32_t* d_col_idx);
+
+/* GPU pseudo-peripheral start search, semantics of SimplePeripheral::findPeripheral (outputnodes = 32). */
+int rcm_ctx_find_start(rcm_ctx* ctx, int32_t* start_out, int32_t* iterations_out);
+
+/* Reorder the loaded matrix from start_node (>= 0). Result stays on the device. */
+int rcm_ctx_run(rcm_ctx* ctx, int32_t start_node, uint32_t flags);
+
+/* Bandwidth of the loaded matrix under the last computed ordering (reversal does not change it). */
+int rcm_ctx_bandwidth(rcm_ctx* ctx, int64_t* bandwidth_out);
+
+/* Bandwidth of the loaded matrix under an external host ordering perm[new] = old (n entries; checked to be a
+ * permutation of [0, n) on the host, RCM_EINVAL otherwise). Replaces computeBandwidth(graph, permutation)
+ * (src/main.cpp:143-184) for orderings produced elsewhere. The ordering becomes the context's result. */
+int rcm_ctx_bandwidth_of(rcm_ctx* ctx, const int32_t* perm, int64_t* bandwidth_out);
+
+/* Copy the last ordering to the host (n entries), reversed when RCM_FLAG_REVERSE was set in rcm_ctx_run. */
+int rcm_ctx_read_perm(rcm_ctx* ctx, int32_t* perm_out);
+/* Device pointer to the last ordering (n entries, same orientation as rcm_ctx_read_perm). */
+int rcm_ctx_perm_device(rcm_ctx* ctx, const int32_t** d_perm_out);
+
+typedef struct rcm_timing {
+  double run_kernel_ms;     /* persistent reorder kernel only (the reference's timed region) */
+  double init_kernel_ms;    /* state reset kernel */
+  double start_search_ms;   /* last rcm_ctx_find_start, all iterations */
+  double bandwidth_ms;      /* last rcm_ctx_bandwidth */
+  double load_ms;           /* last rcm_ctx_load: H2D copy + validation */
+  int32_t workers;          /* workgroups of the last run */
+  int32_t batches;          /* speculative batches of the last run */
+  int32_t start_iterations; /* BFS sweeps of the last start search */
+  int32_t placed;           /* nodes placed by the last run */
+  int32_t error_site;       /* device-side diagnostics of the last failed run (0 = none) */
+  int32_t error_info[8];
+} rcm_timing_t;
+int rcm_ctx_timing(rcm_ctx* ctx, rcm_timing_t* out);
+
+/* ---- many matrices in one persistent launch ---- */
+/* Reorders `count` independent matrices stored back to back on the host:
+ *   row_ptr_cat + rp_off[k] holds matrix k's n[k]+1 local offsets, col_idx_cat + ci_off[k] its nnz[k] columns.
+ * start[k] < 0 picks the pseudo-peripheral start for matrix k. perm_cat + po_off[k] receives n[k] entries.
+ * status_out[k] (nullable) receives per-matrix status; the return value is RCM_OK if all succeeded. */
+int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, const int64_t* rp_off,
+                       const int64_t* ci_off, const int32_t* row_ptr_cat, const int32_t* col_idx_cat,
+                       const int32_t* start, uint32_t flags, const int64_t* po_off, int32_t* perm_cat,
+                       int64_t* bandwidth_out, int32_t* status_out);
+
+const char* rcm_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

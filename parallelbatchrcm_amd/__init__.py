@@ -1,0 +1,455 @@
+"""parallelbatchrcm_amd -- MI355X-native Cuthill-McKee reordering (drop-in for the GPU_BATCH path).
+
+Python host layer over the C ABI in include/rcm_gpu_batch.h. It mirrors the reference's host interface for
+this path (include/rcm_gpu_batch.h:40-41, include/simple_peripheral.h:45-46, src/main.cpp:143-226):
+
+    GPU_BATCH.rcm(graph, csv_file, permutation, start_node)   -> CM order on the GPU
+    SimplePeripheral.findPeripheral(graph, csv_file, writeOutput, outputnodes=32)
+    computeBandwidth(graph, permutation), checkPermutation(permutation)
+
+plus a device-resident `RcmContext`, the one-shot `rcm_gpu_batch`, the batched `rcm_gpu_batch_many`, and the
+seeded workload generators (`workloads`). Every compute call goes to the HIP engine; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import I32P, I64P, CsrT, TimingT
+
+RCM_FLAG_REVERSE = 1
+RCM_FLAG_STABLE = 2
+RCM_FLAG_BANDWIDTH = 4
+
+RCM_OK = 0
+RCM_EINVAL = -1
+RCM_ENOTCONNECTED = -2
+RCM_ENOMEM = -3
+RCM_EDEVICE = -4
+RCM_ETIMEOUT = -5
+RCM_ECAPACITY = -6
+RCM_ENOTLOADED = -7
+
+PARAM_WORKERS = 1
+PARAM_TIMEOUT_MS = 2
+PARAM_SORT = 3
+PARAM_DIGIT_BITS = 4
+PARAM_VARIANT = 5
+PARAM_STRICT = 6
+
+
+class RcmError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        msg = _native.lib().rcm_status_string(status).decode()
+        super().__init__(f"{what}: {msg} ({status})" if what else f"{msg} ({status})")
+        self.status = status
+
+
+def _check(status: int, what: str) -> None:
+    if status != RCM_OK:
+        raise RcmError(status, what)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(I32P)
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(I64P)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Containers mirroring include/CSR.h:42-75 and include/Graph.h:42-70
+# ---------------------------------------------------------------------------------------------------------
+@dataclass
+class CSR:
+    """CSR<T> (include/CSR.h:42-75): rows, cols, nnz, row_offsets[rows+1], col_ids[nnz], data[nnz]."""
+
+    row_offsets: np.ndarray
+    col_ids: np.ndarray
+    data: Optional[np.ndarray] = None
+    cols: int = -1
+
+    def __post_init__(self):
+        self.row_offsets = _i32(self.row_offsets)
+        self.col_ids = _i32(self.col_ids)
+        if self.cols < 0:
+            self.cols = self.rows
+
+    @property
+    def rows(self) -> int:
+        return int(self.row_offsets.shape[0]) - 1
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_ids.shape[0])
+
+
+@dataclass
+class Graph:
+    """Graph<T> (include/Graph.h:42-333): the CSR plus the host helpers the RCM path uses."""
+
+    csr: CSR
+
+    @property
+    def size(self) -> int:
+        return self.csr.cols
+
+    def neighbour_count(self, node: int) -> int:  # Graph.h:58-61
+        return int(self.csr.row_offsets[node + 1] - self.csr.row_offsets[node])
+
+    def prepareRCM(self, check_sym: bool = True) -> int:
+        """Graph<T>::prepareRCM (Graph.h:74-264): symmetrise (optional) and connect components.
+
+        Returns the number of connecting edges added. Raises RcmError(RCM_EINVAL) where the reference's
+        repair loop would not terminate (a component whose lowest node has no entries).
+        """
+        m = self.csr
+        if check_sym:
+            m = symmetrize(m)
+        m, added = connect_components(m)
+        self.csr = m
+        return added
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Workload generators and host canonicalisation (include/rcm_workloads.h)
+# ---------------------------------------------------------------------------------------------------------
+def _take_csr(t: CsrT) -> CSR:
+    n, nnz = t.n, t.nnz
+    rp = np.ctypeslib.as_array(t.row_ptr, shape=(n + 1,)).copy()
+    ci = np.ctypeslib.as_array(t.col_idx, shape=(max(nnz, 1),))[:nnz].copy() if nnz else np.zeros(0, np.int32)
+    _native.lib().rcm_csr_free(C.byref(t))
+    return CSR(rp, ci)
+
+
+def _as_csr_t(m: CSR) -> CsrT:
+    return CsrT(m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids))
+
+
+class workloads:
+    """Seeded synthetic inputs (splitmix64 + Fisher-Yates; deterministic across hosts)."""
+
+    @staticmethod
+    def stencil2d(nx: int, ny: int) -> CSR:
+        t = CsrT()
+        _check(_native.lib().rcm_gen_stencil2d(nx, ny, C.byref(t)), "stencil2d")
+        return _take_csr(t)
+
+    @staticmethod
+    def stencil3d(nx: int, ny: int, nz: int) -> CSR:
+        t = CsrT()
+        _check(_native.lib().rcm_gen_stencil3d(nx, ny, nz, C.byref(t)), "stencil3d")
+        return _take_csr(t)
+
+    @staticmethod
+    def rgg(npts: int, mean_degree: float, seed: int) -> CSR:
+        t = CsrT()
+        _check(_native.lib().rcm_gen_rgg(npts, float(mean_degree), seed & (2**64 - 1), C.byref(t)), "rgg")
+        return _take_csr(t)
+
+    @staticmethod
+    def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> CSR:
+        t = CsrT()
+        _check(_native.lib().rcm_gen_rmat(scale, edge_factor, a, b, c, seed & (2**64 - 1), C.byref(t)), "rmat")
+        return _take_csr(t)
+
+    @staticmethod
+    def permute_symmetric(m: CSR, seed: int):
+        """B = P A P^T with a seeded Fisher-Yates P; returns (B, q) where node i of A is node q[i] of B."""
+        t = CsrT()
+        q = np.empty(m.rows, np.int32)
+        src = _as_csr_t(m)
+        _check(_native.lib().rcm_permute_symmetric(C.byref(src), seed & (2**64 - 1), C.byref(t), _p32(q)), "permute")
+        return _take_csr(t), q
+
+    @staticmethod
+    def config(which: int, scale: int = 0, seed: int = 0) -> CSR:
+        """Benchmark configurations: 1 = 2D 5-point 100^2, 2 = 3D 7-point 128^3, 3 = RGG 10^6 points deg 12.
+        `scale` shrinks them for tests (side length, or number of points)."""
+        t = CsrT()
+        _check(_native.lib().rcm_gen_config(which, scale, seed & (2**64 - 1), C.byref(t)), f"config {which}")
+        return _take_csr(t)
+
+    @staticmethod
+    def batch_mix(count: int, n_lo: int = 1000, n_hi: int = 200000, seed: int = 4, threads: int = 0):
+        """Batched mix (stencils and RGGs of log-uniform size). Returns a BatchSet."""
+        lib = _native.lib()
+        ns, nnzs, rpo, cio, rpc, cic = I32P(), I64P(), I64P(), I64P(), I32P(), I32P()
+        _check(
+            lib.rcm_gen_batch_mix(count, n_lo, n_hi, seed & (2**64 - 1), threads, C.byref(ns), C.byref(nnzs),
+                                  C.byref(rpo), C.byref(cio), C.byref(rpc), C.byref(cic)),
+            "batch_mix",
+        )
+        n = np.ctypeslib.as_array(ns, shape=(count,)).copy()
+        nnz = np.ctypeslib.as_array(nnzs, shape=(count,)).copy()
+        rp_off = np.ctypeslib.as_array(rpo, shape=(count,)).copy()
+        ci_off = np.ctypeslib.as_array(cio, shape=(count,)).copy()
+        trp = int(rp_off[-1] + n[-1] + 1)
+        tci = int(ci_off[-1] + nnz[-1])
+        row_ptr = np.ctypeslib.as_array(rpc, shape=(trp,)).copy()
+        col_idx = np.ctypeslib.as_array(cic, shape=(max(tci, 1),))[:tci].copy()
+        for p in (ns, nnzs, rpo, cio, rpc, cic):
+            lib.rcm_free(C.cast(p, C.c_void_p))
+        return BatchSet(n.astype(np.int32), nnz.astype(np.int64), rp_off.astype(np.int64), ci_off.astype(np.int64),
+                        row_ptr, col_idx)
+
+
+@dataclass
+class BatchSet:
+    """Matrices stored back to back: matrix k has n[k] rows, local row_ptr at row_ptr[rp_off[k]:...]."""
+
+    n: np.ndarray
+    nnz: np.ndarray
+    rp_off: np.ndarray
+    ci_off: np.ndarray
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.n.shape[0])
+
+    def matrix(self, k: int) -> CSR:
+        n, z = int(self.n[k]), int(self.nnz[k])
+        rp = self.row_ptr[self.rp_off[k]: self.rp_off[k] + n + 1]
+        ci = self.col_idx[self.ci_off[k]: self.ci_off[k] + z]
+        return CSR(rp, ci)
+
+    @property
+    def perm_offsets(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.n.astype(np.int64))[:-1]]).astype(np.int64)
+
+
+def symmetrize(m: CSR) -> CSR:
+    t = CsrT()
+    src = _as_csr_t(m)
+    r = _native.lib().rcm_symmetrize(C.byref(src), C.byref(t))
+    if r < 0:
+        raise RcmError(RCM_EINVAL, "symmetrize")
+    return _take_csr(t)
+
+
+def connect_components(m: CSR):
+    """Connectivity repair of prepareRCM (Graph.h:160-263). Returns (csr, edges_added)."""
+    t = CsrT()
+    src = _as_csr_t(m)
+    r = _native.lib().rcm_connect_components(C.byref(src), C.byref(t))
+    if r < 0:
+        raise RcmError(RCM_EINVAL, "connect_components (the reference repair loop would not terminate)")
+    return _take_csr(t), int(r)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Device-resident context
+# ---------------------------------------------------------------------------------------------------------
+@dataclass
+class Timing:
+    run_kernel_ms: float
+    init_kernel_ms: float
+    start_search_ms: float
+    bandwidth_ms: float
+    load_ms: float
+    workers: int
+    batches: int
+    start_iterations: int
+    placed: int
+    error_site: int = 0
+    error_info: tuple = ()
+
+
+class RcmContext:
+    """Owns device buffers for one matrix at a time (rcm_ctx in include/rcm_gpu_batch.h)."""
+
+    def __init__(self, device: int = 0, **params):
+        self._lib = _native.lib()
+        h = C.c_void_p()
+        _check(self._lib.rcm_ctx_create(device, C.byref(h)), "rcm_ctx_create")
+        self._h = h
+        self.n = 0
+        for k, v in params.items():
+            self.set_param(k, v)
+
+    _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT,
+               "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT}
+
+    def set_param(self, name: str, value: int) -> None:
+        _check(self._lib.rcm_ctx_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
+
+    def get_param(self, name: str) -> int:
+        v = C.c_int64()
+        _check(self._lib.rcm_ctx_get_param(self._h, self._PARAMS[name], C.byref(v)), f"get_param {name}")
+        return int(v.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.rcm_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, m: CSR) -> None:
+        self._rp, self._ci = m.row_offsets, m.col_ids  # keep host arrays alive during the copy
+        _check(self._lib.rcm_ctx_load(self._h, m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids)), "load")
+        self.n = m.rows
+
+    def find_start(self) -> int:
+        s, it = C.c_int32(), C.c_int32()
+        _check(self._lib.rcm_ctx_find_start(self._h, C.byref(s), C.byref(it)), "find_start")
+        return int(s.value)
+
+    def run(self, start: int, flags: int = RCM_FLAG_STABLE) -> None:
+        rc = self._lib.rcm_ctx_run(self._h, int(start), flags)
+        if rc != RCM_OK:
+            t = self.timing()
+            raise RcmError(rc, f"run (device diagnostics: site {t.error_site}, {t.error_info})")
+
+    def run_status(self, start: int, flags: int = RCM_FLAG_STABLE) -> int:
+        return int(self._lib.rcm_ctx_run(self._h, int(start), flags))
+
+    def bandwidth(self) -> int:
+        bw = C.c_int64()
+        _check(self._lib.rcm_ctx_bandwidth(self._h, C.byref(bw)), "bandwidth")
+        return int(bw.value)
+
+    def bandwidth_of(self, perm) -> int:
+        """Bandwidth of the loaded matrix under an external ordering perm[new] = old (validated)."""
+        p = _i32(perm)
+        bw = C.c_int64()
+        _check(self._lib.rcm_ctx_bandwidth_of(self._h, _p32(p), C.byref(bw)), "bandwidth_of")
+        return int(bw.value)
+
+    def perm(self) -> np.ndarray:
+        out = np.empty(self.n, np.int32)
+        _check(self._lib.rcm_ctx_read_perm(self._h, _p32(out)), "read_perm")
+        return out
+
+    def timing(self) -> Timing:
+        t = TimingT()
+        _check(self._lib.rcm_ctx_timing(self._h, C.byref(t)), "timing")
+        return Timing(t.run_kernel_ms, t.init_kernel_ms, t.start_search_ms, t.bandwidth_ms, t.load_ms, t.workers,
+                      t.batches, t.start_iterations, t.placed, t.error_site, tuple(t.error_info))
+
+
+def rcm_gpu_batch(m: CSR, start_node: int = -1, flags: int = RCM_FLAG_STABLE):
+    """One-shot call of the C ABI. Returns (perm, bandwidth or None, start_used)."""
+    perm = np.empty(m.rows, np.int32)
+    bw = C.c_int64(-1)
+    st = C.c_int32(-1)
+    _check(
+        _native.lib().rcm_gpu_batch(m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids), int(start_node), flags,
+                                    _p32(perm), C.byref(bw), C.byref(st)),
+        "rcm_gpu_batch",
+    )
+    return perm, (int(bw.value) if flags & RCM_FLAG_BANDWIDTH else None), int(st.value)
+
+
+def rcm_gpu_batch_many(bs: BatchSet, starts: Optional[Sequence[int]] = None, flags: int = RCM_FLAG_STABLE):
+    """Reorder every matrix of a BatchSet. Returns (perm_cat, per-matrix offsets, bandwidths, statuses)."""
+    k = len(bs)
+    po = bs.perm_offsets
+    perm = np.empty(int(bs.n.astype(np.int64).sum()), np.int32)
+    bw = np.full(k, -1, np.int64)
+    st = np.zeros(k, np.int32)
+    s = _i32(starts if starts is not None else np.full(k, -1))
+    rc = _native.lib().rcm_gpu_batch_many(
+        k, _p32(bs.n), _p64(bs.nnz), _p64(bs.rp_off), _p64(bs.ci_off), _p32(bs.row_ptr), _p32(bs.col_idx),
+        _p32(s), flags, _p64(po), _p32(perm), _p64(bw), _p32(st))
+    return perm, po, bw, st, int(rc)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Reference-shaped entry points
+# ---------------------------------------------------------------------------------------------------------
+class GPU_BATCH:
+    """Mirror of namespace GPU_BATCH (include/rcm_gpu_batch.h:37-45)."""
+
+    warmups = 20  # src/device/rcm_gpu_batch.cu:47-48
+    runs = 20
+
+    @classmethod
+    def rcm(cls, graph: Graph, csv_file, permutation: list, start_node: int) -> None:
+        """GPU_BATCH::rcm (src/device/rcm_gpu_batch.cu:42-112): CM order (not reversed) into `permutation`
+        (resized to n), mean kernel milliseconds of `runs` timed launches after `warmups` written to csv_file."""
+        with RcmContext() as ctx:
+            ctx.load(graph.csr)
+            total = 0.0
+            for r in range(cls.warmups + cls.runs):
+                ctx.run(start_node, RCM_FLAG_STABLE)
+                if r >= cls.warmups:
+                    total += ctx.timing().run_kernel_ms
+            mean = total / max(cls.runs, 1)
+            print(f"{'GPU-DYN-RCM':>27} duration: {mean} ms")
+            if csv_file is not None:
+                csv_file.write(f"{mean}")
+                csv_file.flush()
+            permutation[:] = ctx.perm().tolist()
+
+    @staticmethod
+    def findPeripheral(graph: Graph, csv_file=None, writeOutput: bool = False) -> int:
+        return SimplePeripheral.findPeripheral(graph, csv_file, writeOutput)
+
+
+class SimplePeripheral:
+    """Mirror of namespace SimplePeripheral (include/simple_peripheral.h:39-47), computed on the GPU."""
+
+    @staticmethod
+    def findPeripheral(graph: Graph, csv_file=None, writeOutput: bool = False, outputnodes: int = 32) -> int:
+        if outputnodes != 32:
+            raise ValueError("the GPU start search implements outputnodes = 32 (the reference default)")
+        with RcmContext() as ctx:
+            ctx.load(graph.csr)
+            s = ctx.find_start()
+            if writeOutput and csv_file is not None:
+                csv_file.write(f",{ctx.timing().start_search_ms}")
+                csv_file.flush()
+            return s
+
+
+def computeBandwidth(graph: Graph, permutation) -> int:
+    """computeBandwidth (src/main.cpp:143-184) of an arbitrary full ordering, as a GPU reduction.
+    Returns -1 for an empty permutation (as the reference)."""
+    p = _i32(permutation)
+    if p.size == 0:
+        return -1
+    with RcmContext() as ctx:
+        ctx.load(graph.csr)
+        return ctx.bandwidth_of(p)
+
+
+def checkPermutation(permutation) -> int:
+    """checkPermutation (src/main.cpp:186-226) as a strict bijection check: 0 if valid, -1 otherwise."""
+    p = np.asarray(permutation, dtype=np.int64)
+    n = p.shape[0]
+    if n == 0:
+        return 0
+    if p.min() < 0 or p.max() >= n:
+        return -1
+    seen = np.zeros(n, np.int8)
+    seen[p] = 1
+    return 0 if int(seen.sum()) == n else -1
+
+
+__all__ = [
+    "CSR", "Graph", "BatchSet", "workloads", "symmetrize", "connect_components", "RcmContext", "Timing",
+    "rcm_gpu_batch", "rcm_gpu_batch_many", "GPU_BATCH", "SimplePeripheral", "computeBandwidth",
+    "checkPermutation", "RcmError",
+]

@@ -1,0 +1,107 @@
+"""ctypes bindings of the engine library (include/rcm_gpu_batch.h, include/rcm_workloads.h).
+
+The library is built in-tree by parallelbatchrcm_amd/build.py. Loading fails loudly when it is missing:
+there is no CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbrcm.so")
+
+I32P = C.POINTER(C.c_int32)
+I64P = C.POINTER(C.c_int64)
+U64P = C.POINTER(C.c_uint64)
+
+
+class CsrT(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz", C.c_int64), ("row_ptr", I32P), ("col_idx", I32P)]
+
+
+class TimingT(C.Structure):
+    _fields_ = [
+        ("run_kernel_ms", C.c_double),
+        ("init_kernel_ms", C.c_double),
+        ("start_search_ms", C.c_double),
+        ("bandwidth_ms", C.c_double),
+        ("load_ms", C.c_double),
+        ("workers", C.c_int32),
+        ("batches", C.c_int32),
+        ("start_iterations", C.c_int32),
+        ("placed", C.c_int32),
+        ("error_site", C.c_int32),
+        ("error_info", C.c_int32 * 8),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    # rcm_gpu_batch.h
+    "rcm_gpu_batch": (C.c_int, [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_uint32, I32P, I64P, I32P]),
+    "rcm_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rcm_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "rcm_ctx_set_param": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+    "rcm_ctx_get_param": (C.c_int, [C.c_void_p, C.c_int, I64P]),
+    "rcm_ctx_load": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, I32P, I32P]),
+    "rcm_ctx_load_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p]),
+    "rcm_ctx_find_start": (C.c_int, [C.c_void_p, I32P, I32P]),
+    "rcm_ctx_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32]),
+    "rcm_ctx_bandwidth": (C.c_int, [C.c_void_p, I64P]),
+    "rcm_ctx_bandwidth_of": (C.c_int, [C.c_void_p, I32P, I64P]),
+    "rcm_ctx_read_perm": (C.c_int, [C.c_void_p, I32P]),
+    "rcm_ctx_perm_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "rcm_ctx_timing": (C.c_int, [C.c_void_p, C.POINTER(TimingT)]),
+    "rcm_gpu_batch_many": (
+        C.c_int,
+        [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],
+    ),
+    "rcm_status_string": (C.c_char_p, [C.c_int]),
+    # rcm_workloads.h
+    "rcm_csr_free": (None, [C.POINTER(CsrT)]),
+    "rcm_gen_stencil2d": (C.c_int32, [C.c_int32, C.c_int32, C.POINTER(CsrT)]),
+    "rcm_gen_stencil3d": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(CsrT)]),
+    "rcm_gen_rgg": (C.c_int32, [C.c_int32, C.c_double, C.c_uint64, C.POINTER(CsrT)]),
+    "rcm_gen_rmat": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(CsrT)],
+    ),
+    "rcm_permute_symmetric": (C.c_int32, [C.POINTER(CsrT), C.c_uint64, C.POINTER(CsrT), I32P]),
+    "rcm_connect_components": (C.c_int64, [C.POINTER(CsrT), C.POINTER(CsrT)]),
+    "rcm_symmetrize": (C.c_int64, [C.POINTER(CsrT), C.POINTER(CsrT)]),
+    "rcm_gen_config": (C.c_int32, [C.c_int32, C.c_int64, C.c_uint64, C.POINTER(CsrT)]),
+    "rcm_gen_batch_mix": (
+        C.c_int32,
+        [
+            C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int32,
+            C.POINTER(I32P), C.POINTER(I64P), C.POINTER(I64P), C.POINTER(I64P), C.POINTER(I32P), C.POINTER(I32P),
+        ],
+    ),
+    "rcm_free": (None, [C.c_void_p]),
+    "rcm_splitmix64": (C.c_uint64, [U64P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"engine library {LIB_PATH} is missing; build it with "
+                "`python -m parallelbatchrcm_amd.build` (there is no CPU fallback)"
+            )
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)

@@ -1,0 +1,139 @@
+// device_common.hpp -- gfx950 building blocks shared by the reorder kernels: cross-workgroup memory
+// ordering, wave64 ballot/prefix primitives and block-wide scans.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rcm {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------------------------------------------
+// Cross-workgroup communication.
+//
+// Everything one workgroup writes for another (queue entries, permutation slots, batch records, status
+// words, claim words) is written with agent-scope relaxed atomic stores and read with agent-scope relaxed
+// atomic loads. On gfx950 these compile to global_store/global_load with the sc1 bit: they are coherent
+// across the eight XCD L2 caches without any cache maintenance. A producer orders its data before the
+// flag that publishes it with `publish_wait()` (s_waitcnt: every store of the wave has been acknowledged)
+// in every wave, a workgroup barrier, and then the flag store. A consumer issues its data loads after it
+// has observed the flag (the loads are control-dependent on it).
+//
+// This avoids buffer_wbl2 / buffer_inv, which a formal release/acquire at agent scope emits and which cost
+// 0.5-9 us each on MI355X. RCM_STRICT_FENCES selects the formal fences instead (used by tests to show both
+// give the same results).
+// ------------------------------------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T ld_sync(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_sync(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool STRICT>
+__device__ __forceinline__ void publish_wait() {
+  if constexpr (STRICT)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  else
+    __builtin_amdgcn_s_waitcnt(0);
+}
+template <bool STRICT>
+__device__ __forceinline__ void consume_fence() {
+  if constexpr (STRICT) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+__device__ __forceinline__ uint64_t atomic_min_u64(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t atomic_add_u64(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// 100 MHz constant-rate clock (independent of shader clock), used by the watchdog.
+__device__ __forceinline__ uint64_t wall_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+// ------------------------------------------------------------------------------------------------------
+// Wave64 primitives.
+// ------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / kWave); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, kWave);
+    if (lane_id() >= o) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; `wtmp` needs W+1 words of LDS.
+// Returns the exclusive prefix; *total receives the block sum. Contains two barriers.
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtmp, uint32_t* total) {
+  constexpr int W = T / kWave;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) wtmp[wave_id()] = inc;
+  __syncthreads();
+  uint32_t base = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint32_t s = wtmp[w];
+    base += (w < wave_id()) ? s : 0u;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return base + inc - v;
+}
+
+// Block-wide reduction (max) of one value per thread; `wtmp` needs W words. Two barriers.
+template <int T>
+__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* wtmp) {
+  constexpr int W = T / kWave;
+  v = wave_max(v);
+  if (lane_id() == 0) wtmp[wave_id()] = v;
+  __syncthreads();
+  uint32_t r = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) r = max(r, wtmp[w]);
+  __syncthreads();
+  return r;
+}
+template <int T>
+__device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t* wtmp) {
+  constexpr int W = T / kWave;
+  v = wave_min(v);
+  if (lane_id() == 0) wtmp[wave_id()] = v;
+  __syncthreads();
+  uint32_t r = 0xFFFFFFFFu;
+#pragma unroll
+  for (int w = 0; w < W; ++w) r = min(r, wtmp[w]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ uint32_t bits_for(uint32_t maxval) {  // bits needed to hold 0..maxval
+  return maxval == 0 ? 0u : 32u - (uint32_t)__clz(maxval);
+}
+
+}  // namespace rcm

@@ -1,0 +1,667 @@
+// engine.hpp -- the speculative Cuthill-McKee reorder engine for gfx950 (device code).
+//
+// Algorithm (the speculative batch scheme of the reference's GPU_BATCH path,
+// include/device/batchWorker.cuh:199-280 and include/device/batchQueue.cuh:37-192; re-designed here):
+//
+//   The permutation array is the BFS queue. A *batch* is a range [s, e) of queue positions whose nodes are
+//   expanded together by one workgroup. Batch ids are allocated in queue order, so "earlier batch" means
+//   "parents earlier in the queue". Persistent workgroups take batch ids from a ticket counter in order and
+//   wait until the batch exists (its range record is published by the batch that produced its nodes).
+//
+//   Claims. The parent at queue position p claims its k-th row entry c with code (p << 32 | k) by a 64-bit
+//   atomicMin on claim[c]. The smallest code wins, which is exactly the serial rule: a node belongs to the
+//   earliest-dequeued parent whose row holds it, at that row's first occurrence (src/rcm_cpu.cpp:83-92).
+//   Claims are speculative: a batch may claim before earlier batches have, and lose a node to them later.
+//
+//   Commit. Each batch publishes one 64-bit status word that moves through
+//       DISC_LOCAL (own claims done) -> DISC_INC (all claims up to here done)
+//       -> AGG (own child count, own child-batch count) -> INC (inclusive prefix of both).
+//   Successors read up to 64 predecessor words per wave load (decoupled look-back). A batch whose
+//   predecessors are all claim-complete before it claims ("early") owns every node its atomicMin won, after
+//   an in-LDS removal of duplicates within the batch; otherwise it re-reads its claim words once the
+//   look-back shows every predecessor has claimed.
+//
+//   Order. Owned children are sorted stably by (parent slot, child degree). Slots are generated in
+//   (parent, row index) order, so this equals the serial per-parent std::stable_sort by valence
+//   (src/rcm_cpu.cpp:94-96). The sort is an LDS radix sort whose ranks come from wave64 ballots, or, when
+//   every parent row is short, a per-segment rank count; both are stable and give identical output.
+//
+//   Output. The INC prefix gives the batch's output offset and its first child-batch id. Children are
+//   written to the queue with their row bounds (so the consumer needs no row_ptr lookup), then cut into
+//   child batches (<= P parents, and either <= S adjacency slots or a single node of degree > S/2), whose
+//   range records are published. A 64-bit progress word (nodes written | total) detects completion; the
+//   total becomes known at the batch after which no further batch can exist.
+#pragma once
+#include "device_common.hpp"
+
+namespace rcm {
+
+// ---- status word: state (3 bits) | A (31 bits) | B (30 bits) -------------------------------------------
+enum : uint32_t { ST_NONE = 0, ST_DLOC = 1, ST_DINC = 2, ST_AGG = 3, ST_INC = 4 };
+__device__ __forceinline__ uint64_t mk_status(uint32_t st, uint32_t a, uint32_t b) {
+  return ((uint64_t)st << 61) | ((uint64_t)(a & 0x7FFFFFFFu) << 30) | (uint64_t)(b & 0x3FFFFFFFu);
+}
+__device__ __forceinline__ uint32_t st_state(uint64_t w) { return (uint32_t)(w >> 61); }
+__device__ __forceinline__ uint32_t st_a(uint64_t w) { return (uint32_t)((w >> 30) & 0x7FFFFFFFu); }
+__device__ __forceinline__ uint32_t st_b(uint64_t w) { return (uint32_t)(w & 0x3FFFFFFFu); }
+
+enum : uint32_t { ERR_NONE = 0, ERR_TIMEOUT = 1, ERR_CAPACITY = 2, ERR_LONGROW = 3 };
+
+struct Control {
+  uint32_t ticket;    // next batch id handed to a worker
+  uint32_t done;      // set once when the run is complete or aborted
+  uint32_t error;     // ERR_*
+  uint32_t nbatches;  // batch records used by the finished run
+  uint64_t progress;  // low 32: queue entries written; high 32: total to be written (0 = not yet known)
+  uint32_t placed;    // nodes placed by the finished run
+  uint32_t diag[9];   // first error: site, batch, begin, end, slots, offset, count, first child id, children
+};
+
+struct RunArgs {
+  const int32_t* row_ptr;
+  const int32_t* col_idx;
+  int32_t* perm;          // queue; CM order on completion
+  uint64_t* qinfo;        // per queue position: row_begin | row_end << 32
+  uint64_t* claim;        // per node claim code
+  uint64_t* bstat;        // per batch status word
+  uint64_t* brange;       // per batch: begin | end << 32 (0 = not yet created)
+  int32_t* level;         // per queue position BFS level (BFS mode only)
+  Control* ctl;
+  int32_t n;
+  int32_t bcap;           // records in bstat / brange
+  uint64_t budget_ticks;  // watchdog budget in 100 MHz ticks
+  int32_t sort_mode;      // 0 auto, 1 radix, 2 segment rank
+  int32_t seg_threshold;  // auto mode: segment rank when every parent row has at most this many entries
+};
+
+// Batch geometry: T threads, at most P parents and S adjacency slots per batch.
+template <int T_, int S_, int P_>
+struct Geo {
+  static constexpr int T = T_, S = S_, P = P_;
+  static constexpr int W = T / kWave;
+  static constexpr int EPT = S / T;
+  static constexpr uint32_t Q = S / 2;  // degree quantum of the child-batch cut rule
+  static_assert(S % T == 0 && P <= T && S <= 65536, "geometry");
+};
+
+template <class G, int DIGIT>
+struct Smem {
+  static constexpr int H = (1 << DIGIT) * G::EPT * G::W;
+  static constexpr int HS = H > G::T ? H : G::T;
+  int32_t pnode[G::P];
+  uint32_t prb[G::P];
+  int32_t plevel[G::P];
+  uint32_t so[G::P + 1];
+  int32_t sc[G::S];    // child id, -1 when not owned
+  uint32_t srb[G::S];  // child row begin
+  uint32_t sre[G::S];  // child row end
+  uint16_t si[G::S];   // parent slot of each adjacency slot
+  uint64_t key[2][G::S];
+  uint16_t val[2][G::S];
+  uint32_t hist[HS];
+  uint32_t wtmp[2 * G::W + 2];
+  int32_t b, s, e, early, quit;
+  uint32_t cnt, nb, excl_off, excl_nb, maxdeg, maxpdeg;
+};
+
+__device__ __forceinline__ void raise_error(const RunArgs& a, uint32_t err, uint32_t site = 0, uint32_t v0 = 0,
+                                            uint32_t v1 = 0, uint32_t v2 = 0, uint32_t v3 = 0, uint32_t v4 = 0,
+                                            uint32_t v5 = 0, uint32_t v6 = 0, uint32_t v7 = 0) {
+  uint32_t expected = ERR_NONE;
+  if (__hip_atomic_compare_exchange_strong(&a.ctl->error, &expected, err, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)) {
+    const uint32_t d[9] = {site, v0, v1, v2, v3, v4, v5, v6, v7};
+    for (int i = 0; i < 9; ++i) st_sync(&a.ctl->diag[i], d[i]);
+  }
+  st_sync(&a.ctl->done, 1u);
+}
+
+// Called by every lane of one wave. Returns true when the run must stop (done, or the watchdog fired).
+__device__ __forceinline__ bool should_quit(const RunArgs& a, uint64_t deadline) {
+  if (ld_sync(&a.ctl->done)) return true;
+  if (wall_ticks() > deadline) {
+    if (lane_id() == 0) raise_error(a, ERR_TIMEOUT);
+    return true;
+  }
+  return false;
+}
+
+// ---- look-back over predecessor status words (one wave) -----------------------------------------------
+// Claim completeness of every batch < b, without waiting. Lane l inspects batch (base - l); a position
+// below 0 is the virtual predecessor of batch 0, which counts as complete.
+__device__ __forceinline__ bool claims_complete_before(const RunArgs& a, int64_t b) {
+  int64_t base = b - 1;
+  for (;;) {
+    const int64_t idx = base - lane_id();
+    const uint32_t st = idx >= 0 ? st_state(ld_sync(&a.bstat[idx])) : (uint32_t)ST_DINC;
+    const uint64_t incm = ballot(st >= ST_DINC);
+    const uint64_t notm = ballot(st < ST_DLOC);
+    if (incm) {
+      const uint64_t first = incm & (~incm + 1);
+      return (notm & (first - 1)) == 0;
+    }
+    if (notm) return false;
+    base -= kWave;
+  }
+}
+
+// Blocking form; returns false if the run stopped while waiting.
+__device__ __forceinline__ bool wait_claims_complete(const RunArgs& a, int64_t b, uint64_t deadline) {
+  for (uint32_t spin = 0;; ++spin) {
+    if (claims_complete_before(a, b)) return true;
+    if (should_quit(a, deadline)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Exclusive prefix (queue offset, child-batch id) of batch b. The virtual predecessor of batch 0 is an
+// inclusive record (1, 1): the start node occupies queue slot 0 and batch id 0.
+__device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uint64_t deadline, uint32_t* off,
+                                                uint32_t* nbid) {
+  int64_t base = b - 1;
+  uint32_t acc_off = 0, acc_nb = 0;
+  for (;;) {
+    const int64_t idx = base - lane_id();
+    const uint64_t w = idx >= 0 ? ld_sync(&a.bstat[idx]) : mk_status(ST_INC, 1u, 1u);
+    const uint32_t st = st_state(w);
+    const uint64_t incm = ballot(st == ST_INC);
+    const uint64_t notm = ballot(st < ST_AGG);
+    if (incm) {
+      const int first = __ffsll((unsigned long long)incm) - 1;
+      if ((notm & ((1ull << first) - 1ull)) == 0) {
+        const bool take = lane_id() <= first;
+        acc_off += wave_sum(take ? st_a(w) : 0u);
+        acc_nb += wave_sum(take ? st_b(w) : 0u);
+        *off = acc_off;
+        *nbid = acc_nb;
+        return true;
+      }
+    } else if (notm == 0) {
+      acc_off += wave_sum(st_a(w));
+      acc_nb += wave_sum(st_b(w));
+      base -= kWave;
+      continue;
+    }
+    if (should_quit(a, deadline)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Largest i in [0, m) with so[i] <= j (so[0] == 0 <= j).
+__device__ __forceinline__ int parent_of_slot(const uint32_t* so, int m, uint32_t j) {
+  int lo = 0, hi = m;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (so[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- stable LDS radix sort of (key, val) pairs, cnt <= S ---------------------------------------------
+// Keys: low word = child degree (dbits significant), high word = parent slot (ibits significant). Passes
+// run over the low word's digits, then the high word's. Elements are laid out striped (element e is held by
+// thread e % T in round e / T), so element order is (round, wave, lane); ranks within a wave come from
+// DIGIT ballots, and the histogram is scanned in (digit, round, wave) order, which keeps ties in input order.
+template <class G, int DIGIT>
+__device__ int lds_radix_sort(Smem<G, DIGIT>& sm, uint32_t cnt, uint32_t dbits, uint32_t ibits) {
+  constexpr int R = 1 << DIGIT, EPT = G::EPT, W = G::W, T = G::T;
+  constexpr int H = Smem<G, DIGIT>::H;
+  constexpr int PER = (H + T - 1) / T;
+  const int tid = threadIdx.x, wv = wave_id();
+  const int rounds = (int)((cnt + T - 1) / T);
+  const int npl = (int)((dbits + DIGIT - 1) / DIGIT), nph = (int)((ibits + DIGIT - 1) / DIGIT);
+  int cur = 0;
+  for (int p = 0; p < npl + nph; ++p) {
+    const int shift = p < npl ? p * DIGIT : 32 + (p - npl) * DIGIT;
+    for (int h = tid; h < H; h += T) sm.hist[h] = 0;
+    __syncthreads();
+    uint64_t kk[EPT];
+    uint16_t vv[EPT];
+    uint32_t dg[EPT], rk[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      kk[r] = 0; vv[r] = 0; dg[r] = 0; rk[r] = 0;
+      if (r < rounds) {
+        const uint32_t e = (uint32_t)(r * T + tid);
+        const bool act = e < cnt;
+        const uint64_t k = act ? sm.key[cur][e] : 0ull;
+        const uint32_t d = (uint32_t)(k >> shift) & (uint32_t)(R - 1);
+        uint64_t peers = ballot(act);
+#pragma unroll
+        for (int bit = 0; bit < DIGIT; ++bit) {
+          const bool on = (d >> bit) & 1u;
+          const uint64_t bm = ballot(on);
+          peers &= on ? bm : ~bm;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
+        if (act && rank == 0) sm.hist[(d * EPT + r) * W + wv] = (uint32_t)__popcll(peers);
+        kk[r] = k;
+        vv[r] = act ? sm.val[cur][e] : (uint16_t)0;
+        dg[r] = d;
+        rk[r] = rank;
+      }
+    }
+    __syncthreads();
+    // exclusive scan of hist in index order: thread t owns entries [t*PER, t*PER+PER)
+    uint32_t loc[PER];
+    uint32_t lsum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int h = tid * PER + q;
+      loc[q] = h < H ? sm.hist[h] : 0u;
+      lsum += loc[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<T>(lsum, sm.wtmp, &tot);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int h = tid * PER + q;
+      if (h < H) sm.hist[h] = run;
+      run += loc[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      if (r < rounds) {
+        const uint32_t e = (uint32_t)(r * T + tid);
+        if (e < cnt) {
+          const uint32_t pos = sm.hist[(dg[r] * EPT + r) * W + wv] + rk[r];
+          sm.key[cur ^ 1][pos] = kk[r];
+          sm.val[cur ^ 1][pos] = vv[r];
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// ---- stable segment-rank sort --------------------------------------------------------------------------
+// Compacted elements of one parent are contiguous (compaction keeps slot order). Each element counts the
+// elements of its own segment that order before it by (degree, position) and moves to segment start + rank.
+template <class G, int DIGIT>
+__device__ int lds_segment_sort(Smem<G, DIGIT>& sm, uint32_t cnt) {
+  const int tid = threadIdx.x;
+  for (uint32_t e = tid; e < cnt; e += G::T) {
+    const uint64_t k = sm.key[0][e];
+    const uint32_t seg = (uint32_t)(k >> 32);
+    uint32_t lo = e;
+    while (lo > 0 && (uint32_t)(sm.key[0][lo - 1] >> 32) == seg) --lo;
+    uint32_t rank = 0;
+    for (uint32_t q = lo; q < cnt; ++q) {
+      const uint64_t kq = sm.key[0][q];
+      if ((uint32_t)(kq >> 32) != seg) break;
+      rank += (kq < k || (kq == k && q < e)) ? 1u : 0u;
+    }
+    sm.key[1][lo + rank] = k;
+    sm.val[1][lo + rank] = sm.val[0][e];
+  }
+  __syncthreads();
+  return 1;
+}
+
+// ---- one batch ------------------------------------------------------------------------------------------
+template <class G, int DIGIT, bool STRICT, bool BFS>
+__device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline) {
+  constexpr int T = G::T, S = G::S, EPT = G::EPT;
+  const int tid = threadIdx.x;
+  const uint32_t b = (uint32_t)sm.b;
+  const int32_t s = sm.s, e = sm.e;
+  const int m = e - s;
+  if (m <= 0 || m > G::P) {
+    if (tid == 0) { raise_error(a, ERR_CAPACITY, 1, b, (uint32_t)s, (uint32_t)e); sm.quit = 1; }
+    __syncthreads();
+    return;
+  }
+
+  // A. parents (written by the producing batch with sc1 stores) and the non-blocking claim check
+  uint32_t d = 0;
+  if (tid < m) {
+    const int32_t p = ld_sync(&a.perm[s + tid]);
+    const uint64_t q = ld_sync(&a.qinfo[s + tid]);
+    const uint32_t rb = (uint32_t)q, re = (uint32_t)(q >> 32);
+    sm.pnode[tid] = p;
+    sm.prb[tid] = rb;
+    d = re - rb;
+    if constexpr (BFS) sm.plevel[tid] = ld_sync(&a.level[s + tid]);
+  }
+  if (wave_id() == 0) {
+    const bool ok = claims_complete_before(a, (int64_t)b);
+    if (lane_id() == 0) sm.early = ok ? 1 : 0;
+  }
+  uint32_t D;
+  const uint32_t soff = block_excl_scan<T>(d, sm.wtmp, &D);
+  const uint32_t maxpdeg = block_max<T>(d, sm.wtmp);
+  if (tid < m) sm.so[tid] = soff;
+  if (tid == 0) sm.so[m] = D;
+  __syncthreads();
+  if (D > (uint32_t)S) {
+    if (tid == 0) { raise_error(a, ERR_LONGROW, 2, b, (uint32_t)s, (uint32_t)e, D); sm.quit = 1; }
+    __syncthreads();
+    return;
+  }
+  const bool early = sm.early != 0;
+
+  // B. claim every adjacency slot (striped: thread t owns slots t, t+T, ...). Staged so that all column
+  //    loads, then all atomics and row-bound loads of a thread are in flight together.
+  uint64_t code[EPT], oldv[EPT];
+  int32_t cc[EPT];
+  uint32_t crb[EPT], cre[EPT];
+  uint16_t ci16[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    code[k] = 0;
+    cc[k] = 0;
+    ci16[k] = 0;
+    const uint32_t j = (uint32_t)(tid + k * T);
+    if (j < D) {
+      const int i = parent_of_slot(sm.so, m, j);
+      const uint32_t r = j - sm.so[i];
+      cc[k] = a.col_idx[sm.prb[i] + r];
+      code[k] = ((uint64_t)(uint32_t)(s + i) << 32) | (uint64_t)r;
+      ci16[k] = (uint16_t)i;
+    }
+  }
+  // One wait for every column load: afterwards the atomics below issue back to back (without it the
+  // waitcnt pass cannot count loads across the guarded blocks and waits for everything before each atomic).
+  __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    oldv[k] = 0;
+    crb[k] = 0;
+    cre[k] = 0;
+    if ((uint32_t)(tid + k * T) < D) {
+      oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
+      crb[k] = (uint32_t)a.row_ptr[cc[k]];
+      cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t j = (uint32_t)(tid + k * T);
+    if (j < D) {
+      sm.sc[j] = oldv[k] > code[k] ? cc[k] : -1;
+      sm.srb[j] = crb[k];
+      sm.sre[j] = cre[k];
+      sm.si[j] = ci16[k];
+    }
+  }
+  __syncthreads();
+
+  if (early) {
+    // every earlier batch had finished claiming: a won atomicMin is final unless a slot of this same batch
+    // with a smaller code took the node afterwards; that slot saw our code as its old value.
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t j = (uint32_t)(tid + k * T);
+      if (j < D && oldv[k] > code[k]) {
+        const uint32_t ohi = (uint32_t)(oldv[k] >> 32);
+        if (oldv[k] != ~0ull && ohi >= (uint32_t)s && ohi < (uint32_t)e)
+          sm.sc[sm.so[ohi - (uint32_t)s] + (uint32_t)oldv[k]] = -1;
+      }
+    }
+    publish_wait<STRICT>();
+    __syncthreads();
+    if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DINC, 0, 0));
+  } else {
+    publish_wait<STRICT>();
+    __syncthreads();
+    if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DLOC, 0, 0));
+    if (wave_id() == 0) {
+      const bool ok = wait_claims_complete(a, (int64_t)b, deadline);
+      if (lane_id() == 0) sm.quit = ok ? 0 : 1;
+    }
+    __syncthreads();
+    if (sm.quit) return;
+    consume_fence<STRICT>();
+    if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DINC, 0, 0));
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t j = (uint32_t)(tid + k * T);
+      if (j < D && sm.sc[j] >= 0) {
+        if (ld_sync(&a.claim[sm.sc[j]]) != code[k]) sm.sc[j] = -1;
+      }
+    }
+    __syncthreads();
+  }
+
+  // C. compact owned children in slot order (blocked: thread t scans slots [t*EPT, t*EPT+EPT))
+  uint32_t lc = 0, lmax = 0;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t j = (uint32_t)(tid * EPT + k);
+    if (j < D && sm.sc[j] >= 0) {
+      ++lc;
+      lmax = max(lmax, sm.sre[j] - sm.srb[j]);
+    }
+  }
+  uint32_t cnt;
+  uint32_t pos = block_excl_scan<T>(lc, sm.wtmp, &cnt);
+  const uint32_t maxdeg = block_max<T>(lmax, sm.wtmp);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t j = (uint32_t)(tid * EPT + k);
+    if (j < D && sm.sc[j] >= 0) {
+      sm.key[0][pos] = ((uint64_t)sm.si[j] << 32) | (uint64_t)(sm.sre[j] - sm.srb[j]);
+      sm.val[0][pos] = (uint16_t)j;
+      ++pos;
+    }
+  }
+  __syncthreads();
+
+  // D. order: plain BFS keeps slot order; CM sorts by (parent slot, degree), stably
+  int fin = 0;
+  if constexpr (!BFS) {
+    if (cnt > 1) {
+      const bool seg = a.sort_mode == 2 || (a.sort_mode == 0 && maxpdeg <= (uint32_t)a.seg_threshold);
+      if (seg)
+        fin = lds_segment_sort<G, DIGIT>(sm, cnt);
+      else
+        fin = lds_radix_sort<G, DIGIT>(sm, cnt, bits_for(maxdeg), bits_for((uint32_t)(m - 1)));
+    }
+  }
+
+  // E. cut into child batches: a child starts a new batch at a count quantum (P), at a degree quantum (Q),
+  //    or next to a child of degree > Q; every batch then has <= P parents and <= S slots or one node.
+  uint64_t* cut = sm.key[fin ^ 1];
+  uint32_t ldeg[EPT];
+  uint32_t lsum = 0;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t o = (uint32_t)(tid * EPT + k);
+    ldeg[k] = 0;
+    if (o < cnt) {
+      const uint32_t j = sm.val[fin][o];
+      ldeg[k] = sm.sre[j] - sm.srb[j];
+    }
+    lsum += ldeg[k];
+  }
+  uint32_t dtot;
+  uint32_t dx = block_excl_scan<T>(lsum, sm.wtmp, &dtot);
+  uint32_t lstarts = 0;
+  uint32_t sflag = 0;  // bit k: element tid*EPT+k starts a batch
+  {
+    uint32_t prev_dx = 0, prev_deg = 0;
+    const uint32_t o0 = (uint32_t)(tid * EPT);
+    if (o0 > 0 && o0 < cnt) {
+      const uint32_t jp = sm.val[fin][o0 - 1];
+      prev_deg = sm.sre[jp] - sm.srb[jp];
+      prev_dx = dx - prev_deg;
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t o = o0 + (uint32_t)k;
+      if (o < cnt) {
+        const bool st = o == 0 || (dx / G::Q) != (prev_dx / G::Q) || (o / G::P) != ((o - 1) / G::P) ||
+                        ldeg[k] > G::Q || prev_deg > G::Q;
+        if (st) { sflag |= 1u << k; ++lstarts; }
+        prev_dx = dx;
+        prev_deg = ldeg[k];
+        dx += ldeg[k];
+      }
+    }
+  }
+  uint32_t nb;
+  uint32_t bidx = block_excl_scan<T>(lstarts, sm.wtmp, &nb);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k)
+    if (sflag & (1u << k)) cut[bidx++] = (uint64_t)(tid * EPT + k);
+
+  // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record
+  if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_AGG, cnt, nb));
+  if (wave_id() == 0) {
+    uint32_t xo = 0, xb = 0;
+    const bool ok = lookback_prefix(a, (int64_t)b, deadline, &xo, &xb);
+    if (lane_id() == 0) {
+      sm.quit = ok ? 0 : 1;
+      sm.excl_off = xo;
+      sm.excl_nb = xb;
+      if (ok) st_sync(&a.bstat[b], mk_status(ST_INC, xo + cnt, xb + nb));
+    }
+  }
+  __syncthreads();
+  if (sm.quit) return;
+  const uint32_t xo = sm.excl_off, xb = sm.excl_nb;
+  if (xo + cnt > (uint32_t)a.n || xb + nb > (uint32_t)a.bcap) {
+    if (tid == 0) { raise_error(a, ERR_CAPACITY, 3, b, (uint32_t)s, (uint32_t)e, D, xo, cnt, xb, nb); sm.quit = 1; }
+    __syncthreads();
+    return;
+  }
+
+  // G. write children (queue entry, row bounds, level), then publish the child batch records
+  for (uint32_t o = tid; o < cnt; o += T) {
+    const uint32_t j = sm.val[fin][o];
+    const uint32_t p = xo + o;
+    st_sync(&a.perm[p], sm.sc[j]);
+    st_sync(&a.qinfo[p], (uint64_t)sm.srb[j] | ((uint64_t)sm.sre[j] << 32));
+    if constexpr (BFS) st_sync(&a.level[p], sm.plevel[sm.si[j]] + 1);
+  }
+  publish_wait<STRICT>();
+  __syncthreads();
+  for (uint32_t t = tid; t < nb; t += T) {
+    const uint32_t beg = (uint32_t)cut[t];
+    const uint32_t end = t + 1 < nb ? (uint32_t)cut[t + 1] : cnt;
+    st_sync(&a.brange[xb + t], (uint64_t)(xo + beg) | ((uint64_t)(xo + end) << 32));
+  }
+  if (tid == 0) {
+    const bool final_batch = xb + nb == b + 1;  // no batch beyond b exists or can be created
+    if (cnt > 0 || final_batch) {
+      const uint64_t add = (uint64_t)cnt + (final_batch ? ((uint64_t)(xo + cnt) << 32) : 0ull);
+      if (final_batch) {
+        st_sync(&a.ctl->nbatches, b + 1);
+        st_sync(&a.ctl->placed, xo + cnt);
+      }
+      const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
+      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) st_sync(&a.ctl->done, 1u);
+    }
+  }
+}
+
+// ---- the persistent kernel --------------------------------------------------------------------------------
+template <class G, int DIGIT, bool STRICT, bool BFS>
+__global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
+  __shared__ Smem<G, DIGIT> sm;
+  const int tid = threadIdx.x;
+  const uint64_t deadline = wall_ticks() + a.budget_ticks;
+  for (;;) {
+    if (tid == 0) {
+      const uint32_t b = atomic_add_u32(&a.ctl->ticket, 1u);
+      int quit = 0;
+      uint64_t r = 0;
+      for (uint32_t spin = 0;; ++spin) {
+        if (b < (uint32_t)a.bcap) {
+          r = ld_sync(&a.brange[b]);
+          if (r) break;
+        }
+        if (ld_sync(&a.ctl->done)) { quit = 1; break; }
+        if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      sm.quit = quit;
+      sm.b = (int32_t)b;
+      sm.s = (int32_t)(uint32_t)r;
+      sm.e = (int32_t)(r >> 32);
+    }
+    __syncthreads();
+    if (sm.quit) return;
+    consume_fence<STRICT>();
+    process_batch<G, DIGIT, STRICT, BFS>(a, sm, deadline);
+    __syncthreads();
+    const int quit = sm.quit;
+    __syncthreads();  // every wave has read the scalars before thread 0 rewrites them for the next batch
+    if (quit) return;
+  }
+}
+
+// ---- helper kernels -----------------------------------------------------------------------------------------
+// Reset for a run from `start`: claims, the first `bclear` batch records, the control block, queue slot 0.
+__global__ void k_init(RunArgs a, int32_t start, int32_t bclear) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i < a.n; i += stride) a.claim[i] = i == start ? 0ull : ~0ull;
+  for (int64_t i = tid; i < bclear; i += stride) {
+    a.bstat[i] = 0ull;
+    a.brange[i] = i == 0 ? (1ull << 32) : 0ull;  // batch 0 = queue range [0, 1)
+  }
+  if (tid == 0) {
+    a.perm[0] = start;
+    a.qinfo[0] = (uint64_t)(uint32_t)a.row_ptr[start] | ((uint64_t)(uint32_t)a.row_ptr[start + 1] << 32);
+    if (a.level) a.level[0] = 0;
+    Control c = {};
+    c.progress = 1;  // the start node is written
+    *a.ctl = c;
+  }
+}
+
+// CSR validation: rp[0] == 0, rp[n] == nnz, rows non-decreasing, columns in [0, n). out[0] = error flags,
+// out[1] = max row length.
+__global__ void k_validate(const int32_t* rp, const int32_t* ci, int32_t n, int64_t nnz, uint32_t* out) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t bad = 0, mx = 0;
+  for (int64_t i = tid; i < n; i += stride) {
+    const int32_t b = rp[i], e = rp[i + 1];
+    if (e < b) bad |= 1u;
+    else mx = max(mx, (uint32_t)(e - b));
+  }
+  if (tid == 0 && (rp[0] != 0 || (int64_t)rp[n] != nnz)) bad |= 2u;
+  for (int64_t k = tid; k < nnz; k += stride) {
+    const int32_t c = ci[k];
+    if (c < 0 || c >= n) bad |= 4u;
+  }
+  mx = wave_max(mx);
+  if (bad) atomicOr(&out[0], bad);
+  if (lane_id() == 0 && mx) atomicMax(&out[1], mx);
+}
+
+// inv[perm[i]] = i
+__global__ void k_inverse(const int32_t* perm, int32_t n, int32_t* inv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+// computeBandwidth (src/main.cpp:143-184): max over rows with entries of max(inv[i]-min, max-inv[i]),
+// i.e. the largest |inv[i] - inv[j]| over stored entries.
+__global__ void k_bandwidth(const int32_t* rp, const int32_t* ci, const int32_t* inv, int32_t n, uint32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bw = 0;
+  if (i < n) {
+    const int32_t ri = inv[i];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const int32_t v = inv[ci[k]];
+      bw = max(bw, (uint32_t)(v > ri ? v - ri : ri - v));
+    }
+  }
+  bw = wave_max(bw);
+  if (lane_id() == 0 && bw) atomicMax(out, bw);
+}
+
+// out[i] = perm[n - 1 - i]
+__global__ void k_reverse(const int32_t* perm, int32_t n, int32_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = perm[n - 1 - i];
+}
+
+}  // namespace rcm
