@@ -64,7 +64,8 @@ enum rcm_param {
   RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
   RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width: 2, 4 (default) or 6 */
   RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads/2048 slots, 1 = 256/4096, 2 = 512/4096 */
-  RCM_PARAM_STRICT = 6       /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
+  RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
+  RCM_PARAM_PROFILE = 7      /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);
@@ -110,6 +111,10 @@ typedef struct rcm_timing {
   int32_t error_info[8];
 } rcm_timing_t;
 int rcm_ctx_timing(rcm_ctx* ctx, rcm_timing_t* out);
+/* Per-phase profile of the last instrumented run (RCM_PARAM_PROFILE = 1): 100 MHz tick sums over all batches
+ * for acquire, parents, claim, finalise, sort, cut, prefix, write; then counts of batches, early batches,
+ * owned children and adjacency slots. Copies min(count, 16) words. */
+int rcm_ctx_profile(rcm_ctx* ctx, uint64_t* out, int32_t count);
 
 /* ---- many matrices in one persistent launch ---- */
 /* Reorders `count` independent matrices stored back to back on the host:

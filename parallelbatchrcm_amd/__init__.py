@@ -40,6 +40,8 @@ PARAM_SORT = 3
 PARAM_DIGIT_BITS = 4
 PARAM_VARIANT = 5
 PARAM_STRICT = 6
+PARAM_PROFILE = 7
+PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
 class RcmError(RuntimeError):
@@ -279,7 +281,8 @@ class RcmContext:
             self.set_param(k, v)
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT,
-               "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT}
+               "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT,
+               "profile": PARAM_PROFILE}
 
     def set_param(self, name: str, value: int) -> None:
         _check(self._lib.rcm_ctx_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
@@ -340,6 +343,15 @@ class RcmContext:
     def perm(self) -> np.ndarray:
         out = np.empty(self.n, np.int32)
         _check(self._lib.rcm_ctx_read_perm(self._h, _p32(out)), "read_perm")
+        return out
+
+    def profile(self) -> dict:
+        """Per-phase profile of the last instrumented run: microseconds summed over batches per phase,
+        and counts (batches, early, children, slots)."""
+        buf = (C.c_uint64 * 16)()
+        _check(self._lib.rcm_ctx_profile(self._h, buf, 16), "profile")
+        out = {ph: buf[i] * 0.01 for i, ph in enumerate(PROFILE_PHASES)}  # 100 MHz ticks -> us
+        out.update(batches=int(buf[8]), early=int(buf[9]), children=int(buf[10]), slots=int(buf[11]))
         return out
 
     def timing(self) -> Timing:

@@ -53,6 +53,7 @@ _SIGNATURES = {
     "rcm_ctx_read_perm": (C.c_int, [C.c_void_p, I32P]),
     "rcm_ctx_perm_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rcm_ctx_timing": (C.c_int, [C.c_void_p, C.POINTER(TimingT)]),
+    "rcm_ctx_profile": (C.c_int, [C.c_void_p, U64P, C.c_int32]),
     "rcm_gpu_batch_many": (
         C.c_int,
         [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],

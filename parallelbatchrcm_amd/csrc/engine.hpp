@@ -72,6 +72,24 @@ struct RunArgs {
   uint64_t budget_ticks;  // watchdog budget in 100 MHz ticks
   int32_t sort_mode;      // 0 auto, 1 radix, 2 segment rank
   int32_t seg_threshold;  // auto mode: segment rank when every parent row has at most this many entries
+  uint64_t* prof;         // PROF builds: per-phase tick sums (kProfSlots entries)
+};
+
+// Phase profile (PROF builds): 100 MHz ticks summed over all batches, plus event counts.
+enum : int {
+  PF_ACQUIRE = 0,   // ticket taken -> batch record visible
+  PF_PARENTS = 1,   // parent loads, claim check, slot offsets
+  PF_CLAIM = 2,     // column loads, atomics, row bounds
+  PF_FINAL = 3,     // in-batch dedupe (early) or wait for predecessors' claims + revalidation
+  PF_SORT = 4,      // compaction and ordering
+  PF_CUT = 5,       // child-batch cut
+  PF_PREFIX = 6,    // publish aggregate, look back for the prefix
+  PF_WRITE = 7,     // queue writes, child records, progress
+  PF_BATCHES = 8,   // count of batches
+  PF_EARLY = 9,     // count of early batches
+  PF_CHILDREN = 10, // sum of owned children
+  PF_SLOTS = 11,    // sum of adjacency slots
+  kProfSlots = 16
 };
 
 // Batch geometry: T threads, at most P parents and S adjacency slots per batch.
@@ -303,8 +321,8 @@ __device__ int lds_segment_sort(Smem<G, DIGIT>& sm, uint32_t cnt) {
 }
 
 // ---- one batch ------------------------------------------------------------------------------------------
-template <class G, int DIGIT, bool STRICT, bool BFS>
-__device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline) {
+template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF>
+__device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline, uint64_t* pf) {
   constexpr int T = G::T, S = G::S, EPT = G::EPT;
   const int tid = threadIdx.x;
   const uint32_t b = (uint32_t)sm.b;
@@ -344,6 +362,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   }
   const bool early = sm.early != 0;
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PARENTS] += _t - pf[15]; pf[15] = _t; } }
   // B. claim every adjacency slot (striped: thread t owns slots t, t+T, ...). Staged so that all column
   //    loads, then all atomics and row-bound loads of a thread are in flight together.
   uint64_t code[EPT], oldv[EPT];
@@ -390,6 +409,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   }
   __syncthreads();
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CLAIM] += _t - pf[15]; pf[15] = _t; } }
   if (early) {
     // every earlier batch had finished claiming: a won atomicMin is final unless a slot of this same batch
     // with a smaller code took the node afterwards; that slot saw our code as its old value.
@@ -427,6 +447,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     __syncthreads();
   }
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_FINAL] += _t - pf[15]; pf[15] = _t; } }
   // C. compact owned children in slot order (blocked: thread t scans slots [t*EPT, t*EPT+EPT))
   uint32_t lc = 0, lmax = 0;
 #pragma unroll
@@ -463,6 +484,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     }
   }
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_SORT] += _t - pf[15]; pf[15] = _t; } }
   // E. cut into child batches: a child starts a new batch at a count quantum (P), at a degree quantum (Q),
   //    or next to a child of degree > Q; every batch then has <= P parents and <= S slots or one node.
   uint64_t* cut = sm.key[fin ^ 1];
@@ -509,6 +531,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   for (int k = 0; k < EPT; ++k)
     if (sflag & (1u << k)) cut[bidx++] = (uint64_t)(tid * EPT + k);
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CUT] += _t - pf[15]; pf[15] = _t; } }
   // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record
   if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_AGG, cnt, nb));
   if (wave_id() == 0) {
@@ -530,6 +553,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     return;
   }
 
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PREFIX] += _t - pf[15]; pf[15] = _t; } }
   // G. write children (queue entry, row bounds, level), then publish the child batch records
   for (uint32_t o = tid; o < cnt; o += T) {
     const uint32_t j = sm.val[fin][o];
@@ -556,17 +580,31 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
       if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) st_sync(&a.ctl->done, 1u);
     }
+    if constexpr (PROF) {
+      const uint64_t _t = wall_ticks();
+      pf[PF_WRITE] += _t - pf[15];
+      pf[15] = _t;
+      pf[PF_BATCHES] += 1;
+      pf[PF_EARLY] += early ? 1 : 0;
+      pf[PF_CHILDREN] += cnt;
+      pf[PF_SLOTS] += D;
+    }
   }
 }
 
 // ---- the persistent kernel --------------------------------------------------------------------------------
-template <class G, int DIGIT, bool STRICT, bool BFS>
+template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF = false>
 __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   __shared__ Smem<G, DIGIT> sm;
   const int tid = threadIdx.x;
   const uint64_t deadline = wall_ticks() + a.budget_ticks;
+  uint64_t pf[kProfSlots];  // thread 0's phase sums (PROF builds)
+  if constexpr (PROF) {
+    for (int i = 0; i < kProfSlots; ++i) pf[i] = 0;
+  }
   for (;;) {
     if (tid == 0) {
+      if constexpr (PROF) pf[15] = wall_ticks();
       const uint32_t b = atomic_add_u32(&a.ctl->ticket, 1u);
       int quit = 0;
       uint64_t r = 0;
@@ -584,14 +622,25 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
       sm.s = (int32_t)(uint32_t)r;
       sm.e = (int32_t)(r >> 32);
     }
+    if constexpr (PROF) {
+      if (tid == 0) {
+        const uint64_t _t = wall_ticks();
+        if (!sm.quit) pf[PF_ACQUIRE] += _t - pf[15];
+        pf[15] = _t;
+      }
+    }
     __syncthreads();
-    if (sm.quit) return;
+    if (sm.quit) break;
     consume_fence<STRICT>();
-    process_batch<G, DIGIT, STRICT, BFS>(a, sm, deadline);
+    process_batch<G, DIGIT, STRICT, BFS, PROF>(a, sm, deadline, pf);
     __syncthreads();
     const int quit = sm.quit;
     __syncthreads();  // every wave has read the scalars before thread 0 rewrites them for the next batch
-    if (quit) return;
+    if (quit) break;
+  }
+  if constexpr (PROF) {
+    if (tid == 0 && a.prof)
+      for (int i = 0; i < 12; ++i) atomic_add_u64(&a.prof[i], pf[i]);
   }
 }
 

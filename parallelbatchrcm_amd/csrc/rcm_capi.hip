@@ -22,9 +22,9 @@ using GeoSmall = Geo<256, 2048, 256>;
 constexpr int kDigit = 4;
 constexpr int kDefaultSegThreshold = 24;
 
-template <bool STRICT, bool BFS>
+template <bool STRICT, bool BFS, bool PROF = false>
 void* reorder_kernel() {
-  return reinterpret_cast<void*>(&k_reorder<GeoSmall, kDigit, STRICT, BFS>);
+  return reinterpret_cast<void*>(&k_reorder<GeoSmall, kDigit, STRICT, BFS, PROF>);
 }
 
 #define RCM_HIP(call)                         \
@@ -55,6 +55,8 @@ struct rcm_ctx {
   int64_t digit_bits = kDigit;
   int64_t variant = 0;
   int64_t strict = 0;
+  int64_t profile = 0;
+  uint64_t* prof = nullptr;  // kProfSlots words
   // matrix
   int32_t n = 0;
   int64_t nnz = 0;
@@ -145,6 +147,9 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   a.budget_ticks = (uint64_t)c->timeout_ms * 100000ull;
   a.sort_mode = (int32_t)c->sort_mode;
   a.seg_threshold = kDefaultSegThreshold;
+  const bool prof = c->profile && !bfs && !c->strict;
+  a.prof = prof ? c->prof : nullptr;
+  if (prof) RCM_HIP(hipMemsetAsync(c->prof, 0, kProfSlots * sizeof(uint64_t), c->stream));
   // clear every record an earlier run may have written, also beyond this matrix's n (a previous, larger
   // matrix in the same buffers): workers read records ahead of their producers
   const int32_t bclear = std::max(1, std::min(c->dirty, c->cap + 1));
@@ -154,7 +159,8 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   hipLaunchKernelGGL(k_init, dim3(init_blocks), dim3(256), 0, c->stream, a, start, bclear);
   RCM_HIP(hipEventRecord(c->ev[1], c->stream));
   void* kern = c->strict ? (bfs ? reorder_kernel<true, true>() : reorder_kernel<true, false>())
-                         : (bfs ? reorder_kernel<false, true>() : reorder_kernel<false, false>());
+                         : (bfs ? reorder_kernel<false, true>()
+                                : (prof ? reorder_kernel<false, false, true>() : reorder_kernel<false, false>()));
   void* args[] = {&a};
   RCM_HIP(hipLaunchKernel(kern, dim3(workers), dim3(GeoSmall::T), args, 0, c->stream));
   RCM_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -230,7 +236,8 @@ int rcm_ctx_create(int device, rcm_ctx** out) {
   c->device = device;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
-  ok = ok && dev_alloc(&c->ctl, 1) == hipSuccess && dev_alloc(&c->scratch, 16) == hipSuccess;
+  ok = ok && dev_alloc(&c->ctl, 1) == hipSuccess && dev_alloc(&c->scratch, 16) == hipSuccess &&
+       dev_alloc(&c->prof, kProfSlots) == hipSuccess;
   ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(Control), hipHostMallocDefault) == hipSuccess;
   int per_cu = 0;
   hipDeviceProp_t prop;
@@ -264,6 +271,7 @@ int rcm_ctx_destroy(rcm_ctx* c) {
   dev_free(c->inv);
   dev_free(c->ctl);
   dev_free(c->scratch);
+  dev_free(c->prof);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -281,8 +289,17 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_DIGIT_BITS: if (v != kDigit) return RCM_EINVAL; c->digit_bits = v; return RCM_OK;
     case RCM_PARAM_VARIANT: if (v != 0) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_PROFILE: c->profile = v ? 1 : 0; return RCM_OK;
     default: return RCM_EINVAL;
   }
+}
+
+int rcm_ctx_profile(rcm_ctx* c, uint64_t* out, int32_t count) {
+  if (!c || !out || count <= 0) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(c->device));
+  RCM_HIP(hipMemcpy(out, c->prof, (size_t)std::min<int32_t>(count, kProfSlots) * sizeof(uint64_t),
+                    hipMemcpyDeviceToHost));
+  return RCM_OK;
 }
 
 int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
@@ -294,6 +311,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_DIGIT_BITS: *v = c->digit_bits; return RCM_OK;
     case RCM_PARAM_VARIANT: *v = c->variant; return RCM_OK;
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
+    case RCM_PARAM_PROFILE: *v = c->profile; return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
