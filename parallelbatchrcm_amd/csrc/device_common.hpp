@@ -1,8 +1,15 @@
 // device_common.hpp -- gfx950 building blocks shared by the reorder kernels: cross-workgroup memory
 // ordering, wave64 ballot/prefix primitives and block-wide scans.
 #pragma once
+#ifndef RCM_WAVE_EMU  // the CPU wave emulator of the tests provides these builtins itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
+
+// Workgroup-shared (LDS) variable declaration.
+#ifndef RCM_LDS
+#define RCM_LDS(T, name) __shared__ T name
+#endif
 
 namespace rcm {
 

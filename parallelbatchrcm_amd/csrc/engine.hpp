@@ -595,7 +595,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
 // ---- the persistent kernel --------------------------------------------------------------------------------
 template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF = false>
 __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
-  __shared__ Smem<G, DIGIT> sm;
+  using SmemT = Smem<G, DIGIT>;
+  RCM_LDS(SmemT, sm);
   const int tid = threadIdx.x;
   const uint64_t deadline = wall_ticks() + a.budget_ticks;
   uint64_t pf[kProfSlots];  // thread 0's phase sums (PROF builds)

@@ -1,0 +1,62 @@
+// emu_block.cpp -- runs the library's workgroup engine (engine.hpp) under the CPU wave emulator.
+// TEST INFRASTRUCTURE ONLY. Built by tests/emu/Makefile into tests/emu/libemu_block.so.
+#include "wave_emu.hpp"
+#include "engine.hpp"
+
+#include <memory>
+
+using namespace rcm;
+
+namespace {
+// the library's geometry: 256 threads, <= 2048 adjacency slots, <= 256 parents per batch
+using GeoLib = Geo<256, 2048, 256>;
+template <bool BFS>
+void run_reorder(unsigned workers, RunArgs a) {
+  emu::emu_lds_bytes = sizeof(Smem<GeoLib, 4>);
+  emu::launch(workers, GeoLib::T, true, k_reorder<GeoLib, 4, false, BFS, false>, a);
+  emu::emu_lds_bytes = 256 * 1024;
+}
+}  // namespace
+
+extern "C" {
+
+// info[0] = device error, [1] = placed, [2] = batches, [3..11] = error diagnostics
+int emu_run(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_t start, int32_t workers,
+            int32_t slots, int32_t sort_mode, int32_t bfs, int32_t* perm_out, int32_t* level_out, int32_t* info,
+            double timeout_s) {
+  if (n <= 0 || start < 0 || start >= n || workers <= 0) return -1;
+  std::vector<uint64_t> claim(n), qinfo(n), bstat(n + 1, 0), brange(n + 1, 0);
+  std::vector<int32_t> perm(n, -1), level(n, -1);
+  Control ctl{};
+  RunArgs a{};
+  a.row_ptr = rp;
+  a.col_idx = ci;
+  a.perm = perm.data();
+  a.qinfo = qinfo.data();
+  a.claim = claim.data();
+  a.bstat = bstat.data();
+  a.brange = brange.data();
+  a.level = bfs ? level.data() : nullptr;
+  a.ctl = &ctl;
+  a.n = n;
+  a.bcap = n + 1;
+  a.budget_ticks = (uint64_t)(timeout_s * 1e8);
+  a.sort_mode = sort_mode;
+  a.seg_threshold = 24;
+  a.prof = nullptr;
+  emu::launch(1, 64, false, k_init, a, start, n + 1);
+  if (slots != GeoLib::S) return -1;
+  if (bfs)
+    run_reorder<true>(workers, a);
+  else
+    run_reorder<false>(workers, a);
+  std::memcpy(perm_out, perm.data(), sizeof(int32_t) * n);
+  if (level_out) std::memcpy(level_out, level.data(), sizeof(int32_t) * n);
+  info[0] = (int32_t)ctl.error;
+  info[1] = (int32_t)ctl.placed;
+  info[2] = (int32_t)ctl.nbatches;
+  for (int i = 0; i < 9; ++i) info[3 + i] = (int32_t)ctl.diag[i];
+  return 0;
+}
+}
+
