@@ -1,5 +1,5 @@
 /*
- * rcm_gpu_batch.h -- C ABI of the MI355X (gfx950) Cuthill-McKee engine.
+ * rcm_gpu_batch_abi.h -- C ABI of the MI355X (gfx950) Cuthill-McKee engine.
  *
  * Drop-in for the reference's GPU_BATCH path. Every entry point replaces a reference interface, cited as
  * file:line in the reference repository (GPUPeople/ParallelBatchRCM):
@@ -24,8 +24,8 @@
  *     negative rcm_status. A disconnected graph returns RCM_ENOTCONNECTED (the reference kernel hangs).
  *   - Thread safety: one rcm_ctx per host thread; distinct contexts are independent.
  */
-#ifndef RCM_GPU_BATCH_H
-#define RCM_GPU_BATCH_H
+#ifndef RCM_GPU_BATCH_ABI_H
+#define RCM_GPU_BATCH_ABI_H
 #include <stdint.h>
 
 #ifdef __cplusplus

@@ -1,6 +1,6 @@
 """parallelbatchrcm_amd -- MI355X-native Cuthill-McKee reordering (drop-in for the GPU_BATCH path).
 
-Python host layer over the C ABI in include/rcm_gpu_batch.h. It mirrors the reference's host interface for
+Python host layer over the C ABI in include/rcm_gpu_batch_abi.h. It mirrors the reference's host interface for
 this path (include/rcm_gpu_batch.h:40-41, include/simple_peripheral.h:45-46, src/main.cpp:143-226):
 
     GPU_BATCH.rcm(graph, csv_file, permutation, start_node)   -> CM order on the GPU
@@ -269,7 +269,7 @@ class Timing:
 
 
 class RcmContext:
-    """Owns device buffers for one matrix at a time (rcm_ctx in include/rcm_gpu_batch.h)."""
+    """Owns device buffers for one matrix at a time (rcm_ctx in include/rcm_gpu_batch_abi.h)."""
 
     def __init__(self, device: int = 0, **params):
         self._lib = _native.lib()

@@ -1,4 +1,4 @@
-"""ctypes bindings of the engine library (include/rcm_gpu_batch.h, include/rcm_workloads.h).
+"""ctypes bindings of the engine library (include/rcm_gpu_batch_abi.h, include/rcm_workloads.h).
 
 The library is built in-tree by parallelbatchrcm_amd/build.py. Loading fails loudly when it is missing:
 there is no CPU fallback behind this package.
@@ -38,7 +38,7 @@ class TimingT(C.Structure):
 
 # name -> (restype, argtypes)
 _SIGNATURES = {
-    # rcm_gpu_batch.h
+    # rcm_gpu_batch_abi.h
     "rcm_gpu_batch": (C.c_int, [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_uint32, I32P, I64P, I32P]),
     "rcm_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "rcm_ctx_destroy": (C.c_int, [C.c_void_p]),

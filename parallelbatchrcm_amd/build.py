@@ -1,7 +1,7 @@
 """Build the gfx950 engine library in-tree: parallelbatchrcm_amd/libpbrcm.so.
 
 One hipcc invocation compiles the HIP engine (csrc/*.hip) and the host C++ workload generators
-(csrc/*.cpp) into a single shared library exporting the C ABI of include/rcm_gpu_batch.h and
+(csrc/*.cpp) into a single shared library exporting the C ABI of include/rcm_gpu_batch_abi.h and
 include/rcm_workloads.h.
 """
 from __future__ import annotations

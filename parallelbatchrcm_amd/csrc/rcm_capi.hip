@@ -1,4 +1,4 @@
-// rcm_capi.hip -- host side of the C ABI (include/rcm_gpu_batch.h): device-resident contexts, kernel
+// rcm_capi.hip -- host side of the C ABI (include/rcm_gpu_batch_abi.h): device-resident contexts, kernel
 // launches, the GPU pseudo-peripheral start search and the bandwidth reduction.
 #include <hip/hip_runtime.h>
 
@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "engine.hpp"
-#include "rcm_gpu_batch.h"
+#include "rcm_gpu_batch_abi.h"
 
 using namespace rcm;
 
