@@ -11,8 +11,8 @@
  *   rcm_ctx_run / rcm_ctx_load    the device half of GPU_BATCH::rcm (upload, init kernel, persistent run
  *                                 kernel, download; src/device/rcm_gpu_batch.cu:57-102) split so callers
  *                                 can keep the CSR resident in HBM between calls
- *   rcm_gpu_batch_many            no single reference counterpart: GPU_BATCH::rcm applied to many matrices,
- *                                 done in one persistent launch
+ *   rcm_batch_* / rcm_gpu_batch_many  no single reference counterpart: GPU_BATCH::rcm applied to many
+ *                                 matrices, done in one persistent launch
  *
  * Conventions
  *   - CSR is int32: row_ptr[n+1], col_idx[nnz]; 1 <= n < 2^30, nnz < 2^31. Values are not needed.
@@ -62,10 +62,11 @@ enum rcm_param {
   RCM_PARAM_WORKERS = 1,     /* persistent workgroups for one matrix (0 = autotuned) */
   RCM_PARAM_TIMEOUT_MS = 2,  /* device watchdog, default 20000 */
   RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
-  RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width: 2, 4 (default) or 6 */
-  RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads/2048 slots, 1 = 256/4096, 2 = 512/4096 */
+  RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width: 4 */
+  RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads, <= 256 parents, <= 2048 adjacency slots */
   RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
-  RCM_PARAM_PROFILE = 7      /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
+  RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
+  RCM_PARAM_GRAPH = 8        /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);
@@ -116,8 +117,53 @@ int rcm_ctx_timing(rcm_ctx* ctx, rcm_timing_t* out);
  * owned children and adjacency slots. Copies min(count, 16) words. */
 int rcm_ctx_profile(rcm_ctx* ctx, uint64_t* out, int32_t count);
 
-/* ---- many matrices in one persistent launch ---- */
-/* Reorders `count` independent matrices stored back to back on the host:
+/* ---- many matrices, device-resident: one persistent launch reorders all of them ---- */
+/* No single reference counterpart: GPU_BATCH::rcm (src/device/rcm_gpu_batch.cu:42-112) applied to every
+ * matrix of a set, with the start of each from SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:92-141)
+ * and its bandwidth from computeBandwidth (src/main.cpp:143-184). Each matrix keeps its own queue, claim words
+ * and batch records; persistent workgroups take whichever matrix has a batch ready. Results are bit-identical
+ * to reordering each matrix alone. */
+typedef struct rcm_batch rcm_batch;
+
+typedef struct rcm_batch_timing {
+  double load_ms;          /* last rcm_batch_load: H2D copies + validation */
+  double start_search_ms;  /* last rcm_batch_find_start, all sweeps */
+  double run_ms;           /* last rcm_batch_run: reset + persistent reorder kernel (device time) */
+  double bandwidth_ms;     /* last rcm_batch_bandwidth (device time) */
+  int32_t count;           /* matrices loaded */
+  int32_t workers;         /* persistent workgroups of the last launch */
+  int32_t start_sweeps;    /* launches of the last start search */
+  int32_t graph_replays;   /* runs served by replaying a captured hipGraph */
+  int64_t batches;         /* speculative batches of the last run, all matrices */
+  int64_t total_n;         /* sum of n */
+  int64_t total_nnz;       /* sum of nnz */
+} rcm_batch_timing_t;
+
+int rcm_batch_create(int device, rcm_batch** out);
+int rcm_batch_destroy(rcm_batch* b);
+/* RCM_PARAM_WORKERS, RCM_PARAM_TIMEOUT_MS, RCM_PARAM_SORT, RCM_PARAM_GRAPH (1 = capture the reset and reorder
+ * launches of rcm_batch_run in a hipGraph and replay it while the loaded set is unchanged; default 1). */
+int rcm_batch_set_param(rcm_batch* b, int param, int64_t value);
+/* Copy `count` matrices to the device (layout as rcm_gpu_batch_many) and validate each; status_out[k]
+ * (nullable) is RCM_OK or RCM_EINVAL for matrix k. Returns RCM_EINVAL if any matrix is malformed. */
+int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t* nnz, const int64_t* rp_off,
+                   const int64_t* ci_off, const int32_t* row_ptr_cat, const int32_t* col_idx_cat,
+                   int32_t* status_out);
+/* Pseudo-peripheral start of every matrix (== SimplePeripheral::findPeripheral per matrix); each sweep is one
+ * persistent launch over the matrices still iterating. */
+int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_out, int32_t* status_out);
+/* Reorder every matrix from start[k] (>= 0). status_out[k] (nullable): RCM_OK, RCM_ENOTCONNECTED, ... */
+int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* status_out);
+/* Bandwidth of every matrix under the last run's order. */
+int rcm_batch_bandwidth(rcm_batch* b, int64_t* bandwidth_out);
+/* Copy every order to the host: matrix k's n[k] entries at perm_cat + po_off[k] (po_off nullable: back to
+ * back in load order). Reversed when the last run had RCM_FLAG_REVERSE. */
+int rcm_batch_read_perm(rcm_batch* b, const int64_t* po_off, int32_t* perm_cat);
+/* Device pointer to matrix k's order (n[k] entries), same orientation as rcm_batch_read_perm. */
+int rcm_batch_perm_device(rcm_batch* b, int32_t k, const int32_t** d_perm_out);
+int rcm_batch_timing(rcm_batch* b, rcm_batch_timing_t* out);
+
+/* One-shot form over host buffers. Reorders `count` independent matrices stored back to back on the host:
  *   row_ptr_cat + rp_off[k] holds matrix k's n[k]+1 local offsets, col_idx_cat + ci_off[k] its nnz[k] columns.
  * start[k] < 0 picks the pseudo-peripheral start for matrix k. perm_cat + po_off[k] receives n[k] entries.
  * status_out[k] (nullable) receives per-matrix status; the return value is RCM_OK if all succeeded. */

@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native
-from ._native import I32P, I64P, CsrT, TimingT
+from ._native import I32P, I64P, BatchTimingT, CsrT, TimingT
 
 RCM_FLAG_REVERSE = 1
 RCM_FLAG_STABLE = 2
@@ -41,6 +41,7 @@ PARAM_DIGIT_BITS = 4
 PARAM_VARIANT = 5
 PARAM_STRICT = 6
 PARAM_PROFILE = 7
+PARAM_GRAPH = 8
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -230,6 +231,18 @@ class BatchSet:
     def perm_offsets(self) -> np.ndarray:
         return np.concatenate([[0], np.cumsum(self.n.astype(np.int64))[:-1]]).astype(np.int64)
 
+    @staticmethod
+    def from_matrices(mats: Sequence) -> "BatchSet":
+        """Pack CSR matrices (or (row_ptr, col_idx) pairs) back to back."""
+        pairs = [(m.row_offsets, m.col_ids) if isinstance(m, CSR) else (m[0], m[1]) for m in mats]
+        n = np.array([len(rp) - 1 for rp, _ in pairs], np.int32)
+        nnz = np.array([len(ci) for _, ci in pairs], np.int64)
+        rp_off = np.concatenate([[0], np.cumsum(n.astype(np.int64) + 1)[:-1]]).astype(np.int64)
+        ci_off = np.concatenate([[0], np.cumsum(nnz)[:-1]]).astype(np.int64)
+        row_ptr = np.concatenate([_i32(rp) for rp, _ in pairs])
+        col_idx = np.concatenate([_i32(ci) for _, ci in pairs] + [np.zeros(0, np.int32)])
+        return BatchSet(n, nnz, rp_off, ci_off, row_ptr, col_idx)
+
 
 def symmetrize(m: CSR) -> CSR:
     t = CsrT()
@@ -386,6 +399,85 @@ def rcm_gpu_batch_many(bs: BatchSet, starts: Optional[Sequence[int]] = None, fla
         k, _p32(bs.n), _p64(bs.nnz), _p64(bs.rp_off), _p64(bs.ci_off), _p32(bs.row_ptr), _p32(bs.col_idx),
         _p32(s), flags, _p64(po), _p32(perm), _p64(bw), _p32(st))
     return perm, po, bw, st, int(rc)
+
+
+class RcmBatch:
+    """Device-resident set of matrices reordered by one persistent launch (rcm_batch in
+    include/rcm_gpu_batch_abi.h). Results are bit-identical to reordering each matrix alone."""
+
+    def __init__(self, device: int = 0, timeout_ms: int = 60000, workers: int = 0, graph: bool = True):
+        h = C.c_void_p()
+        _check(_native.lib().rcm_batch_create(device, C.byref(h)), "rcm_batch_create")
+        self._h = h
+        self.set_param("timeout_ms", timeout_ms)
+        self.set_param("workers", workers)
+        self.set_param("graph", 1 if graph else 0)
+        self._bs: Optional[BatchSet] = None
+
+    _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH}
+
+    def set_param(self, name: str, value: int) -> None:
+        _check(_native.lib().rcm_batch_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
+
+    def load(self, bs: BatchSet) -> np.ndarray:
+        """Upload and validate; returns per-matrix status (raises if any matrix is malformed)."""
+        st = np.zeros(len(bs), np.int32)
+        rc = _native.lib().rcm_batch_load(self._h, len(bs), _p32(bs.n), _p64(bs.nnz), _p64(bs.rp_off),
+                                          _p64(bs.ci_off), _p32(bs.row_ptr), _p32(bs.col_idx), _p32(st))
+        _check(rc, "rcm_batch_load")
+        self._bs = bs
+        return st
+
+    def find_start(self):
+        """Returns (starts, iterations, statuses)."""
+        k = len(self._bs)
+        s, it, st = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(k, np.int32)
+        rc = _native.lib().rcm_batch_find_start(self._h, _p32(s), _p32(it), _p32(st))
+        if rc in (RCM_EDEVICE, RCM_ETIMEOUT, RCM_ENOTLOADED, RCM_EINVAL):
+            raise RcmError(rc, "rcm_batch_find_start")
+        return s, it, st
+
+    def run(self, starts, flags: int = RCM_FLAG_STABLE) -> np.ndarray:
+        """Reorder every matrix; returns per-matrix status (device-level failures raise)."""
+        st = np.zeros(len(self._bs), np.int32)
+        rc = _native.lib().rcm_batch_run(self._h, _p32(_i32(starts)), flags, _p32(st))
+        if rc in (RCM_EDEVICE, RCM_ETIMEOUT, RCM_ENOTLOADED, RCM_EINVAL):
+            raise RcmError(rc, "rcm_batch_run")
+        return st
+
+    def bandwidth(self) -> np.ndarray:
+        bw = np.zeros(len(self._bs), np.int64)
+        _check(_native.lib().rcm_batch_bandwidth(self._h, _p64(bw)), "rcm_batch_bandwidth")
+        return bw
+
+    def perms(self) -> list:
+        bs = self._bs
+        po = bs.perm_offsets
+        cat = np.empty(int(bs.n.astype(np.int64).sum()), np.int32)
+        _check(_native.lib().rcm_batch_read_perm(self._h, _p64(po), _p32(cat)), "rcm_batch_read_perm")
+        return [cat[po[k]: po[k] + int(bs.n[k])] for k in range(len(bs))]
+
+    def timing(self) -> BatchTimingT:
+        t = BatchTimingT()
+        _check(_native.lib().rcm_batch_timing(self._h, C.byref(t)), "rcm_batch_timing")
+        return t
+
+    def close(self) -> None:
+        if self._h:
+            _native.lib().rcm_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------------------------------------

@@ -36,6 +36,22 @@ class TimingT(C.Structure):
     ]
 
 
+class BatchTimingT(C.Structure):
+    _fields_ = [
+        ("load_ms", C.c_double),
+        ("start_search_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("bandwidth_ms", C.c_double),
+        ("count", C.c_int32),
+        ("workers", C.c_int32),
+        ("start_sweeps", C.c_int32),
+        ("graph_replays", C.c_int32),
+        ("batches", C.c_int64),
+        ("total_n", C.c_int64),
+        ("total_nnz", C.c_int64),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     # rcm_gpu_batch_abi.h
@@ -54,6 +70,16 @@ _SIGNATURES = {
     "rcm_ctx_perm_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "rcm_ctx_timing": (C.c_int, [C.c_void_p, C.POINTER(TimingT)]),
     "rcm_ctx_profile": (C.c_int, [C.c_void_p, U64P, C.c_int32]),
+    "rcm_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rcm_batch_destroy": (C.c_int, [C.c_void_p]),
+    "rcm_batch_set_param": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+    "rcm_batch_load": (C.c_int, [C.c_void_p, C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P]),
+    "rcm_batch_find_start": (C.c_int, [C.c_void_p, I32P, I32P, I32P]),
+    "rcm_batch_run": (C.c_int, [C.c_void_p, I32P, C.c_uint32, I32P]),
+    "rcm_batch_bandwidth": (C.c_int, [C.c_void_p, I64P]),
+    "rcm_batch_read_perm": (C.c_int, [C.c_void_p, I64P, I32P]),
+    "rcm_batch_perm_device": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "rcm_batch_timing": (C.c_int, [C.c_void_p, C.POINTER(BatchTimingT)]),
     "rcm_gpu_batch_many": (
         C.c_int,
         [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],

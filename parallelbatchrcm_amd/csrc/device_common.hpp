@@ -68,6 +68,9 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / kWave); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The first active lane's value for every lane. Used on values that are the same in every lane on the device
+// (one wave-wide load of one address, s_memrealtime) so that the control flow they steer is visibly uniform.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 #pragma unroll

@@ -73,7 +73,15 @@ struct RunArgs {
   int32_t sort_mode;      // 0 auto, 1 radix, 2 segment rank
   int32_t seg_threshold;  // auto mode: segment rank when every parent row has at most this many entries
   uint64_t* prof;         // PROF builds: per-phase tick sums (kProfSlots entries)
+  uint32_t* jobs_done;    // many-matrix launches: count of finished matrices (nullptr for one matrix)
 };
+
+// Marks the run finished (complete or failed). The first marker also counts the matrix as finished for a
+// many-matrix launch, so each matrix is counted exactly once.
+__device__ __forceinline__ void mark_done(const RunArgs& a) {
+  const uint32_t was = __hip_atomic_exchange(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (was == 0 && a.jobs_done) atomic_add_u32(a.jobs_done, 1u);
+}
 
 // Phase profile (PROF builds): 100 MHz ticks summed over all batches, plus event counts.
 enum : int {
@@ -131,13 +139,22 @@ __device__ __forceinline__ void raise_error(const RunArgs& a, uint32_t err, uint
     const uint32_t d[9] = {site, v0, v1, v2, v3, v4, v5, v6, v7};
     for (int i = 0; i < 9; ++i) st_sync(&a.ctl->diag[i], d[i]);
   }
-  st_sync(&a.ctl->done, 1u);
+  mark_done(a);
+}
+
+// Every thread of the workgroup: tells the caller to stop after a failed batch. The flag is written only once
+// every thread has passed the barrier, so no wave can still be about to read its previous value.
+template <class SM>
+__device__ __forceinline__ void stop_workgroup(SM& sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) sm.quit = 1;
+  __syncthreads();
 }
 
 // Called by every lane of one wave. Returns true when the run must stop (done, or the watchdog fired).
 __device__ __forceinline__ bool should_quit(const RunArgs& a, uint64_t deadline) {
-  if (ld_sync(&a.ctl->done)) return true;
-  if (wall_ticks() > deadline) {
+  if (wave_uniform(ld_sync(&a.ctl->done))) return true;
+  if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) {
     if (lane_id() == 0) raise_error(a, ERR_TIMEOUT);
     return true;
   }
@@ -320,6 +337,297 @@ __device__ int lds_segment_sort(Smem<G, DIGIT>& sm, uint32_t cnt) {
   return 1;
 }
 
+// ---- one parent whose row exceeds the batch's adjacency slots --------------------------------------------
+// (The reference's discoverLong / constructHist / compactReader, include/device/batchWorker.cuh:627-735,
+// 1430-1651, with an ordered compaction instead of arrival order.)
+//
+// The row is streamed through the workgroup in chunks of S entries, several times:
+//   1. claims: atomicMin(claim[c], s << 32 | k) for every entry k;
+//   2. once every earlier batch has finished claiming, claim[c] is final for this batch: entry k owns c iff
+//      claim[c] still holds its code (later batches only write larger codes) and c is not the parent itself
+//      (the start node's claim word is 0, the code of queue position 0, entry 0). A counting pass gives the
+//      owned count, their degree sum, the number of degree > Q children and the largest degree;
+//   3. the batch publishes AGG with an order-independent upper bound on its child batches, and the look-back
+//      gives its output range [xo, xo + cnt) and first child-batch id xb;
+//   4. the owned children are compacted in row order into the output range and sorted there by degree with
+//      stable LSD passes of DIGIT bits (chunk ranks from ballots, as in lds_radix_sort); the range's perm
+//      and qinfo words serve as the two buffers (qinfo is written last, from the sorted ids);
+//   5. child batches are cut by the same rule as a normal batch, and records beyond the real cuts are
+//      published as empty ranges, which the workers pass through the commit chain without work.
+template <class G, int DIGIT>
+__device__ __forceinline__ uint32_t row_degree(const RunArgs& a, int32_t c) {
+  return (uint32_t)(a.row_ptr[c + 1] - a.row_ptr[c]);
+}
+
+template <class G, int DIGIT, bool STRICT, bool BFS>
+__device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_t b, int32_t s, int32_t parent,
+                                   uint32_t rb, uint32_t D, int32_t plevel, bool early, uint64_t deadline) {
+  constexpr int T = G::T, S = G::S, EPT = G::EPT, W = G::W;
+  constexpr int R = 1 << DIGIT;
+  constexpr int H = Smem<G, DIGIT>::H;  // = R * EPT * W
+  const int tid = threadIdx.x, wv = wave_id();
+  const uint64_t code_hi = (uint64_t)(uint32_t)s << 32;
+
+  // 1. claims, S entries per chunk, loads staged before the atomics
+  for (uint32_t c0 = 0; c0 < D; c0 += S) {
+    int32_t cc[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t j = c0 + (uint32_t)(tid + k * T);
+      cc[k] = j < D ? a.col_idx[rb + j] : 0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t j = c0 + (uint32_t)(tid + k * T);
+      if (j < D) (void)atomic_min_u64(&a.claim[cc[k]], code_hi | j);
+    }
+  }
+  publish_wait<STRICT>();
+  __syncthreads();
+  if (!early) {
+    if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DLOC, 0, 0));
+    if (wave_id() == 0) {
+      const bool ok = wait_claims_complete(a, (int64_t)b, deadline);
+      if (lane_id() == 0) sm.quit = ok ? 0 : 1;
+    }
+    __syncthreads();
+    if (sm.quit) return;
+    consume_fence<STRICT>();
+  }
+  if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DINC, 0, 0));
+
+  // 2. owned children: count, degree sum, big children, largest degree
+  uint32_t lc = 0, ld = 0, lbig = 0, lmax = 0;
+  for (uint32_t j = (uint32_t)tid; j < D; j += T) {
+    const int32_t c = a.col_idx[rb + j];
+    if (c != parent && ld_sync(&a.claim[c]) == (code_hi | j)) {
+      const uint32_t dg = row_degree<G, DIGIT>(a, c);
+      ++lc;
+      ld += dg;
+      lbig += dg > G::Q ? 1u : 0u;
+      lmax = max(lmax, dg);
+    }
+  }
+  uint32_t cnt, dtot, big;
+  (void)block_excl_scan<T>(lc, sm.wtmp, &cnt);
+  (void)block_excl_scan<T>(ld, sm.wtmp, &dtot);
+  (void)block_excl_scan<T>(lbig, sm.wtmp, &big);
+  const uint32_t maxdeg = block_max<T>(lmax, sm.wtmp);
+  uint32_t nbu = 0;  // upper bound of the cut rule's batch count, independent of the order
+  if (cnt > 0) {
+    const uint32_t bound = (cnt + G::P - 1) / G::P + (dtot + G::Q - 1) / G::Q + 2 * big + 1;
+    nbu = min(cnt, bound);
+  }
+
+  // 3. commit the counts, look back for the output range
+  if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_AGG, cnt, nbu));
+  if (wave_id() == 0) {
+    uint32_t xo = 0, xb = 0;
+    const bool ok = lookback_prefix(a, (int64_t)b, deadline, &xo, &xb);
+    if (lane_id() == 0) {
+      sm.quit = ok ? 0 : 1;
+      sm.excl_off = xo;
+      sm.excl_nb = xb;
+      if (ok) st_sync(&a.bstat[b], mk_status(ST_INC, xo + cnt, xb + nbu));
+    }
+  }
+  __syncthreads();
+  if (sm.quit) return;
+  const uint32_t xo = sm.excl_off, xb = sm.excl_nb;
+  if (xo + cnt > (uint32_t)a.n || xb + nbu > (uint32_t)a.bcap) {
+    if (tid == 0) raise_error(a, ERR_CAPACITY, 4, b, (uint32_t)s, (uint32_t)s + 1, D, xo, cnt, xb, nbu);
+    stop_workgroup(sm);
+    return;
+  }
+
+  // 4. compaction in row order, then stable digit passes; buffers: A = perm words, B = qinfo words (as u32)
+  uint32_t* bufA = reinterpret_cast<uint32_t*>(a.perm + xo);
+  uint32_t* bufB = reinterpret_cast<uint32_t*>(a.qinfo + xo);
+  const int passes = BFS ? 0 : (int)((bits_for(maxdeg) + DIGIT - 1) / DIGIT);
+  uint32_t* dst = (passes & 1) ? bufB : bufA;
+  {
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < D; c0 += S) {  // blocked: thread t scans entries c0 + t*EPT .. + EPT - 1
+      int32_t cc[EPT];
+      bool own[EPT];
+      uint32_t n_own = 0;
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const uint32_t j = c0 + (uint32_t)(tid * EPT + k);
+        own[k] = false;
+        cc[k] = 0;
+        if (j < D) {
+          cc[k] = a.col_idx[rb + j];
+          own[k] = cc[k] != parent && ld_sync(&a.claim[cc[k]]) == (code_hi | j);
+          n_own += own[k] ? 1u : 0u;
+        }
+      }
+      uint32_t tot;
+      uint32_t pos = base + block_excl_scan<T>(n_own, sm.wtmp, &tot);
+#pragma unroll
+      for (int k = 0; k < EPT; ++k)
+        if (own[k]) {
+          if constexpr (BFS) {
+            st_sync(&a.perm[xo + pos], cc[k]);
+            st_sync(&a.qinfo[xo + pos], (uint64_t)(uint32_t)a.row_ptr[cc[k]] |
+                                            ((uint64_t)(uint32_t)a.row_ptr[cc[k] + 1] << 32));
+            st_sync(&a.level[xo + pos], plevel + 1);
+          } else {
+            st_sync(&dst[pos], (uint32_t)cc[k]);
+          }
+          ++pos;
+        }
+      base += tot;
+    }
+  }
+  uint32_t* base = reinterpret_cast<uint32_t*>(sm.prb);  // R running digit offsets
+  for (int p = 0; p < passes; ++p) {
+    uint32_t* src = dst;
+    dst = (src == bufA) ? bufB : bufA;
+    const uint32_t shift = (uint32_t)(p * DIGIT);
+    publish_wait<STRICT>();
+    __syncthreads();
+    // digit histogram of the whole list
+    for (int h = tid; h < R; h += T) sm.hist[h] = 0;
+    __syncthreads();
+    for (uint32_t e = (uint32_t)tid; e < cnt; e += T) {
+      const uint32_t dg = (row_degree<G, DIGIT>(a, (int32_t)ld_sync(&src[e])) >> shift) & (uint32_t)(R - 1);
+      atomicAdd(&sm.hist[dg], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int h = 0; h < R; ++h) {
+        base[h] = run;
+        run += sm.hist[h];
+      }
+    }
+    __syncthreads();
+    // chunks of S elements: element e = c0 + r*T + tid; rank among equal digits in (r, wave, lane) order
+    for (uint32_t c0 = 0; c0 < cnt; c0 += S) {
+      for (int h = tid; h < H; h += T) sm.hist[h] = 0;
+      __syncthreads();
+      uint32_t id[EPT], dg[EPT], rk[EPT];
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const uint32_t e = c0 + (uint32_t)(r * T + tid);
+        const bool act = e < cnt;
+        id[r] = act ? ld_sync(&src[e]) : 0u;
+        dg[r] = act ? (row_degree<G, DIGIT>(a, (int32_t)id[r]) >> shift) & (uint32_t)(R - 1) : 0u;
+        uint64_t peers = ballot(act);
+#pragma unroll
+        for (int bit = 0; bit < DIGIT; ++bit) {
+          const bool on = (dg[r] >> bit) & 1u;
+          const uint64_t bm = ballot(on);
+          peers &= on ? bm : ~bm;
+        }
+        rk[r] = (uint32_t)__popcll(peers & lanemask_lt());
+        if (act && rk[r] == 0) sm.hist[(dg[r] * EPT + r) * W + wv] = (uint32_t)__popcll(peers);
+      }
+      __syncthreads();
+      // exclusive scan of the chunk histogram in (digit, round, wave) order
+      constexpr int PER = (H + T - 1) / T;
+      uint32_t loc[PER];
+      uint32_t lsum = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int h = tid * PER + q;
+        loc[q] = h < H ? sm.hist[h] : 0u;
+        lsum += loc[q];
+      }
+      uint32_t ctot;
+      uint32_t run = block_excl_scan<T>(lsum, sm.wtmp, &ctot);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int h = tid * PER + q;
+        if (h < H) sm.hist[h] = run;
+        run += loc[q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const uint32_t e = c0 + (uint32_t)(r * T + tid);
+        if (e < cnt) {
+          const uint32_t d0 = sm.hist[dg[r] * EPT * W];  // start of this digit's elements in the chunk
+          const uint32_t pos = base[dg[r]] + (sm.hist[(dg[r] * EPT + r) * W + wv] - d0) + rk[r];
+          st_sync(&dst[pos], id[r]);
+        }
+      }
+      __syncthreads();
+      if (tid < R) {  // advance the digit offsets by this chunk's counts
+        const uint32_t d0 = sm.hist[tid * EPT * W];
+        const uint32_t d1 = tid + 1 < R ? sm.hist[(tid + 1) * EPT * W] : ctot;
+        base[tid] += d1 - d0;
+      }
+      __syncthreads();
+    }
+  }
+  if constexpr (!BFS) {  // row bounds of the sorted children (qinfo words are free again)
+    publish_wait<STRICT>();
+    __syncthreads();
+    for (uint32_t o = (uint32_t)tid; o < cnt; o += T) {
+      const int32_t c = (int32_t)ld_sync(&bufA[o]);
+      st_sync(&a.qinfo[xo + o], (uint64_t)(uint32_t)a.row_ptr[c] | ((uint64_t)(uint32_t)a.row_ptr[c + 1] << 32));
+    }
+  }
+  publish_wait<STRICT>();
+  __syncthreads();
+
+  // 5. child batches: starts by the cut rule over the final order; records for every start, then padding
+  uint32_t* cs = reinterpret_cast<uint32_t*>(sm.sc);  // starts of the current chunk
+  uint32_t dx_base = 0, prev_deg = 0, nstarts = 0;
+  int32_t pend = -1;  // last start whose end is not yet known
+  for (uint32_t c0 = 0; c0 < cnt; c0 += T) {
+    const uint32_t o = c0 + (uint32_t)tid;
+    const bool act = o < cnt;
+    const uint32_t dg = act ? row_degree<G, DIGIT>(a, ld_sync(&a.perm[xo + o])) : 0u;
+    sm.sre[tid] = dg;
+    uint32_t chunk_deg;
+    const uint32_t dx = dx_base + block_excl_scan<T>(dg, sm.wtmp, &chunk_deg);  // has a barrier after sre
+    const uint32_t pdeg = tid == 0 ? prev_deg : sm.sre[tid - 1];
+    const uint32_t pdx = dx - pdeg;
+    const bool st = act && (o == 0 || (dx / G::Q) != (pdx / G::Q) || (o / G::P) != ((o - 1) / G::P) ||
+                            dg > G::Q || pdeg > G::Q);
+    uint32_t nst;
+    const uint32_t idx = block_excl_scan<T>(st ? 1u : 0u, sm.wtmp, &nst);
+    if (st) cs[idx] = o;
+    __syncthreads();
+    if (nst > 0) {
+      if (tid == 0 && pend >= 0)
+        st_sync(&a.brange[xb + nstarts - 1], (uint64_t)(xo + (uint32_t)pend) | ((uint64_t)(xo + cs[0]) << 32));
+      for (uint32_t t = (uint32_t)tid; t + 1 < nst; t += T)
+        st_sync(&a.brange[xb + nstarts + t], (uint64_t)(xo + cs[t]) | ((uint64_t)(xo + cs[t + 1]) << 32));
+      pend = (int32_t)cs[nst - 1];
+      nstarts += nst;
+    }
+    dx_base += chunk_deg;
+    prev_deg = sm.sre[T - 1];
+    __syncthreads();
+  }
+  if (tid == 0 && pend >= 0)
+    st_sync(&a.brange[xb + nstarts - 1], (uint64_t)(xo + (uint32_t)pend) | ((uint64_t)(xo + cnt) << 32));
+  if (nstarts > nbu) {  // cannot happen: the bound counts every way a start can arise
+    if (tid == 0) raise_error(a, ERR_CAPACITY, 5, b, (uint32_t)s, nstarts, nbu);
+    stop_workgroup(sm);
+    return;
+  }
+  const uint64_t empty = (uint64_t)(xo + cnt) | ((uint64_t)(xo + cnt) << 32);
+  for (uint32_t t = nstarts + (uint32_t)tid; t < nbu; t += T) st_sync(&a.brange[xb + t], empty);
+  if (tid == 0) {
+    const bool final_batch = xb + nbu == b + 1;
+    if (cnt > 0 || final_batch) {
+      const uint64_t add = (uint64_t)cnt + (final_batch ? ((uint64_t)(xo + cnt) << 32) : 0ull);
+      if (final_batch) {
+        st_sync(&a.ctl->nbatches, b + 1);
+        st_sync(&a.ctl->placed, xo + cnt);
+      }
+      const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
+      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) mark_done(a);
+    }
+  }
+}
+
 // ---- one batch ------------------------------------------------------------------------------------------
 template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF>
 __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline, uint64_t* pf) {
@@ -327,10 +635,10 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   const int tid = threadIdx.x;
   const uint32_t b = (uint32_t)sm.b;
   const int32_t s = sm.s, e = sm.e;
-  const int m = e - s;
-  if (m <= 0 || m > G::P) {
-    if (tid == 0) { raise_error(a, ERR_CAPACITY, 1, b, (uint32_t)s, (uint32_t)e); sm.quit = 1; }
-    __syncthreads();
+  const int m = e - s;  // 0 for the padding records a long row may leave (see process_long_batch)
+  if (m < 0 || m > G::P) {
+    if (tid == 0) raise_error(a, ERR_CAPACITY, 1, b, (uint32_t)s, (uint32_t)e);
+    stop_workgroup(sm);
     return;
   }
 
@@ -356,8 +664,14 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   if (tid == 0) sm.so[m] = D;
   __syncthreads();
   if (D > (uint32_t)S) {
-    if (tid == 0) { raise_error(a, ERR_LONGROW, 2, b, (uint32_t)s, (uint32_t)e, D); sm.quit = 1; }
-    __syncthreads();
+    // only a batch of one node can exceed the slots (the cut rule keeps multi-node batches below S)
+    if (m != 1) {
+      if (tid == 0) raise_error(a, ERR_LONGROW, 2, b, (uint32_t)s, (uint32_t)e, D);
+      stop_workgroup(sm);
+      return;
+    }
+    process_long_batch<G, DIGIT, STRICT, BFS>(a, sm, b, s, sm.pnode[0], sm.prb[0], D, BFS ? sm.plevel[0] : 0,
+                                              sm.early != 0, deadline);
     return;
   }
   const bool early = sm.early != 0;
@@ -548,8 +862,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   if (sm.quit) return;
   const uint32_t xo = sm.excl_off, xb = sm.excl_nb;
   if (xo + cnt > (uint32_t)a.n || xb + nb > (uint32_t)a.bcap) {
-    if (tid == 0) { raise_error(a, ERR_CAPACITY, 3, b, (uint32_t)s, (uint32_t)e, D, xo, cnt, xb, nb); sm.quit = 1; }
-    __syncthreads();
+    if (tid == 0) raise_error(a, ERR_CAPACITY, 3, b, (uint32_t)s, (uint32_t)e, D, xo, cnt, xb, nb);
+    stop_workgroup(sm);
     return;
   }
 
@@ -578,7 +892,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
         st_sync(&a.ctl->placed, xo + cnt);
       }
       const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
-      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) st_sync(&a.ctl->done, 1u);
+      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) mark_done(a);
     }
     if constexpr (PROF) {
       const uint64_t _t = wall_ticks();

@@ -11,7 +11,7 @@
 #include <thread>
 #include <vector>
 
-#include "engine.hpp"
+#include "engine_many.hpp"
 #include "rcm_gpu_batch_abi.h"
 
 using namespace rcm;
@@ -205,6 +205,148 @@ int load_common(rcm_ctx* c, int32_t n, int64_t nnz) {
   c->nnz = nnz;
   c->have_perm = false;
   return RCM_OK;
+}
+
+}  // namespace
+
+
+// ---------------------------------------------------------------------------------------------------------
+// Many matrices: device-resident state of a set, one persistent launch per run.
+// ---------------------------------------------------------------------------------------------------------
+struct rcm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  int64_t workers = 0;
+  int64_t timeout_ms = 60000;
+  int64_t sort_mode = 0;
+  int64_t use_graph = 1;
+  int max_resident = 0;
+  // the loaded set
+  int32_t count = 0;
+  std::vector<int32_t> n;
+  std::vector<int64_t> nnz, pn;  // pn[k] = sum_{i<k} n[i]
+  std::vector<int32_t> order;    // all jobs, largest nnz first
+  int64_t total_n = 0, total_nnz = 0;
+  // device buffers (one allocation set per load)
+  int32_t *rp = nullptr, *ci = nullptr;
+  int32_t *perm = nullptr, *level = nullptr, *aux = nullptr;
+  uint64_t *qinfo = nullptr, *claim = nullptr, *bstat = nullptr, *brange = nullptr;
+  Control* ctl = nullptr;
+  JobDesc* jobs = nullptr;
+  int32_t *d_order = nullptr, *d_starts = nullptr, *d_active = nullptr, *tail = nullptr;
+  int64_t* d_nnz = nullptr;
+  uint32_t* scratch = nullptr;  // 3 * count words: validation flags, max row length, bandwidth
+  ManyControl* g = nullptr;
+  // pinned host mirrors
+  Control* h_ctl = nullptr;
+  int32_t* h_tail = nullptr;
+  // graph of the run launches (reset + reorder [+ reverse]), per reverse flag
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t gexec[2] = {};
+  // last result
+  bool have_perm = false;
+  bool reversed = false;
+  std::vector<int32_t> ok_jobs;  // matrices whose last run completed
+  rcm_batch_timing_t timing{};
+};
+
+namespace {
+
+constexpr int kPerJobBlocks = 4;  // helper-kernel workgroups per matrix
+
+void batch_free_set(rcm_batch* b) {
+  for (int r = 0; r < 2; ++r) {
+    if (b->gexec[r]) (void)hipGraphExecDestroy(b->gexec[r]);
+    if (b->graph[r]) (void)hipGraphDestroy(b->graph[r]);
+    b->gexec[r] = nullptr;
+    b->graph[r] = nullptr;
+  }
+  dev_free(b->rp);
+  dev_free(b->ci);
+  dev_free(b->perm);
+  dev_free(b->level);
+  dev_free(b->aux);
+  dev_free(b->qinfo);
+  dev_free(b->claim);
+  dev_free(b->bstat);
+  dev_free(b->brange);
+  dev_free(b->ctl);
+  dev_free(b->jobs);
+  dev_free(b->d_order);
+  dev_free(b->d_starts);
+  dev_free(b->d_active);
+  dev_free(b->tail);
+  dev_free(b->d_nnz);
+  dev_free(b->scratch);
+  dev_free(b->g);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  if (b->h_tail) (void)hipHostFree(b->h_tail);
+  b->h_ctl = nullptr;
+  b->h_tail = nullptr;
+  b->count = 0;
+  b->have_perm = false;
+}
+
+ManyArgs batch_args(rcm_batch* b, const int32_t* d_order, int32_t count) {
+  ManyArgs m{};
+  m.jobs = b->jobs;
+  m.order = d_order;
+  m.starts = b->d_starts;
+  m.count = count;
+  m.g = b->g;
+  m.budget_ticks = (uint64_t)b->timeout_ms * 100000ull;
+  m.sort_mode = (int32_t)b->sort_mode;
+  m.seg_threshold = kDefaultSegThreshold;
+  return m;
+}
+
+int batch_workers(rcm_batch* b) {
+  int w = (int)b->workers;
+  if (w <= 0) w = b->max_resident;
+  return std::max(1, std::min(w, b->max_resident));
+}
+
+void enqueue_many(rcm_batch* b, const int32_t* d_order, int32_t count, bool bfs) {
+  const ManyArgs m = batch_args(b, d_order, count);
+  hipLaunchKernelGGL(k_init_many, dim3((unsigned)count * kPerJobBlocks), dim3(256), 0, b->stream, m, kPerJobBlocks);
+  void* kern = bfs ? reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, true>)
+                   : reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, false>);
+  ManyArgs mm = m;
+  void* args[] = {&mm};
+  (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(GeoSmall::T), args, 0, b->stream);
+}
+
+// Per-matrix status from the control blocks of the last launch (jobs listed in `order`).
+int collect_status(rcm_batch* b, const int32_t* order, int32_t count, int32_t* status, int64_t* batches) {
+  int worst = RCM_OK;
+  int64_t nb = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const int32_t k = order[i];
+    const Control& h = b->h_ctl[k];
+    int r = RCM_OK;
+    if (h.error == ERR_TIMEOUT) r = RCM_ETIMEOUT;
+    else if (h.error == ERR_CAPACITY || h.error == ERR_LONGROW) r = RCM_ECAPACITY;
+    else if (h.error != ERR_NONE || h.nbatches == 0) r = RCM_EDEVICE;
+    else if ((int32_t)h.placed != b->n[k]) r = RCM_ENOTCONNECTED;
+    nb += h.nbatches;
+    if (status) status[k] = r;
+    if (r && worst == RCM_OK) worst = r;
+  }
+  if (batches) *batches = nb;
+  return worst;
+}
+
+int batch_wait(rcm_batch* b, hipEvent_t ev) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipSuccess) return RCM_OK;
+    if (q != hipErrorNotReady) return RCM_EDEVICE;
+    const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
+    if (ms > b->timeout_ms + 10000) return RCM_ETIMEOUT;
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
 }
 
 }  // namespace
@@ -508,6 +650,336 @@ int rcm_gpu_batch(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t*
   return rc;
 }
 
+int rcm_batch_create(int device, rcm_batch** out) {
+  if (!out) return RCM_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return RCM_EDEVICE;
+  RCM_HIP(hipSetDevice(device));
+  rcm_batch* b = new (std::nothrow) rcm_batch();
+  if (!b) return RCM_ENOMEM;
+  b->device = device;
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+  for (auto& e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  hipDeviceProp_t prop;
+  int per_cu = 0;
+  ok = ok && hipGetDeviceProperties(&prop, device) == hipSuccess;
+  ok = ok && hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                 &per_cu, reinterpret_cast<const void*>(&k_reorder_many<GeoSmall, kDigit, false, false>), GeoSmall::T,
+                 0) == hipSuccess;
+  if (!ok) {
+    rcm_batch_destroy(b);
+    return RCM_EDEVICE;
+  }
+  b->max_resident = std::max(1, per_cu) * prop.multiProcessorCount;
+  *out = b;
+  return RCM_OK;
+}
+
+int rcm_batch_destroy(rcm_batch* b) {
+  if (!b) return RCM_EINVAL;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free_set(b);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return RCM_OK;
+}
+
+int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
+  if (!b) return RCM_EINVAL;
+  switch (param) {
+    case RCM_PARAM_WORKERS: if (v < 0) return RCM_EINVAL; b->workers = v; return RCM_OK;
+    case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; b->timeout_ms = v; return RCM_OK;
+    case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; b->sort_mode = v; return RCM_OK;
+    case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
+    default: return RCM_EINVAL;
+  }
+}
+
+int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t* nnz, const int64_t* rp_off,
+                   const int64_t* ci_off, const int32_t* row_ptr_cat, const int32_t* col_idx_cat, int32_t* status_out) {
+  if (!b || count < 1 || !n || !nnz || !rp_off || !ci_off || !row_ptr_cat || !col_idx_cat) return RCM_EINVAL;
+  int64_t rp_end = 0, ci_end = 0, tot = 0, totnnz = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    if (n[k] < 1 || n[k] >= (1 << 30) || nnz[k] < 0 || nnz[k] > INT32_MAX || rp_off[k] < 0 || ci_off[k] < 0)
+      return RCM_EINVAL;
+    rp_end = std::max(rp_end, rp_off[k] + n[k] + 1);
+    ci_end = std::max(ci_end, ci_off[k] + nnz[k]);
+    tot += n[k];
+    totnnz += nnz[k];
+  }
+  RCM_HIP(hipSetDevice(b->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  RCM_HIP(hipStreamSynchronize(b->stream));
+  batch_free_set(b);
+  const size_t N = (size_t)tot, C = (size_t)count;
+  bool ok = dev_alloc(&b->rp, (size_t)rp_end) == hipSuccess && dev_alloc(&b->ci, (size_t)ci_end) == hipSuccess &&
+            dev_alloc(&b->perm, N) == hipSuccess && dev_alloc(&b->level, N) == hipSuccess &&
+            dev_alloc(&b->aux, N) == hipSuccess && dev_alloc(&b->qinfo, N) == hipSuccess &&
+            dev_alloc(&b->claim, N) == hipSuccess && dev_alloc(&b->bstat, N + C) == hipSuccess &&
+            dev_alloc(&b->brange, N + C) == hipSuccess && dev_alloc(&b->ctl, C) == hipSuccess &&
+            dev_alloc(&b->jobs, C) == hipSuccess && dev_alloc(&b->d_order, C) == hipSuccess &&
+            dev_alloc(&b->d_starts, C) == hipSuccess && dev_alloc(&b->d_active, C) == hipSuccess &&
+            dev_alloc(&b->tail, C * 96) == hipSuccess && dev_alloc(&b->d_nnz, C) == hipSuccess &&
+            dev_alloc(&b->scratch, 3 * C) == hipSuccess && dev_alloc(&b->g, 1) == hipSuccess;
+  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&b->h_ctl), C * sizeof(Control), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&b->h_tail), C * 96 * sizeof(int32_t), hipHostMallocDefault) ==
+                 hipSuccess;
+  if (!ok) {
+    batch_free_set(b);
+    return RCM_ENOMEM;
+  }
+  b->count = count;
+  b->n.assign(n, n + count);
+  b->nnz.assign(nnz, nnz + count);
+  b->pn.resize(C);
+  std::vector<JobDesc> jobs(C);
+  int64_t po = 0, pb = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    b->pn[k] = po;
+    JobDesc& J = jobs[k];
+    J.row_ptr = b->rp + rp_off[k];
+    J.col_idx = b->ci + ci_off[k];
+    J.perm = b->perm + po;
+    J.level = b->level + po;
+    J.aux = b->aux + po;
+    J.qinfo = b->qinfo + po;
+    J.claim = b->claim + po;
+    J.bstat = b->bstat + pb;
+    J.brange = b->brange + pb;
+    J.ctl = b->ctl + k;
+    J.n = n[k];
+    J.pad = 0;
+    po += n[k];
+    pb += n[k] + 1;
+  }
+  b->total_n = tot;
+  b->total_nnz = totnnz;
+  // begin the largest matrices first: their chains are the longest
+  b->order.resize(C);
+  for (int32_t k = 0; k < count; ++k) b->order[k] = k;
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int32_t x, int32_t y) { return nnz[x] > nnz[y]; });
+  RCM_HIP(hipMemcpyAsync(b->rp, row_ptr_cat, (size_t)rp_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  if (ci_end > 0)
+    RCM_HIP(hipMemcpyAsync(b->ci, col_idx_cat, (size_t)ci_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->jobs, jobs.data(), C * sizeof(JobDesc), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->d_nnz, nnz, C * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemsetAsync(b->scratch, 0, 3 * C * sizeof(uint32_t), b->stream));
+  hipLaunchKernelGGL(k_validate_many, dim3((unsigned)(C * kPerJobBlocks)), dim3(256), 0, b->stream, b->jobs, count,
+                     kPerJobBlocks, b->d_nnz, b->scratch);
+  std::vector<uint32_t> v(2 * C);
+  RCM_HIP(hipMemcpyAsync(v.data(), b->scratch, 2 * C * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipStreamSynchronize(b->stream));
+  int rc = RCM_OK;
+  for (int32_t k = 0; k < count; ++k) {
+    const int r = v[2 * k] ? RCM_EINVAL : RCM_OK;
+    if (status_out) status_out[k] = r;
+    if (r) rc = r;
+  }
+  b->timing = rcm_batch_timing_t{};
+  b->timing.count = count;
+  b->timing.total_n = tot;
+  b->timing.total_nnz = totnnz;
+  b->timing.load_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (rc) batch_free_set(b);
+  return rc;
+}
+
+// SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:92-141) for every matrix: each sweep is one
+// persistent plain-BFS launch over the matrices still iterating; the rule on the last min(32, n-1) queue
+// entries is applied per matrix on the host, as in rcm_ctx_find_start.
+int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_out, int32_t* status_out) {
+  if (!b || !start_out) return RCM_EINVAL;
+  if (b->count == 0) return RCM_ENOTLOADED;
+  RCM_HIP(hipSetDevice(b->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int32_t C = b->count;
+  std::vector<int32_t> cur(C), iters(C, 0), active, status(C, RCM_OK);
+  std::vector<uint32_t> lastdist(C, 0);
+  for (int32_t k : b->order) {
+    if (b->n[k] == 1) {
+      cur[k] = 0;
+      iters[k] = 1;
+    } else {
+      cur[k] = b->n[k] / 2;
+      active.push_back(k);
+    }
+  }
+  int sweeps = 0;
+  while (!active.empty()) {
+    ++sweeps;
+    const int32_t A = (int32_t)active.size();
+    RCM_HIP(hipMemcpyAsync(b->d_starts, cur.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    RCM_HIP(hipMemcpyAsync(b->d_active, active.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    enqueue_many(b, b->d_active, A, true);
+    hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
+    RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+    RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    RCM_HIP(hipEventRecord(b->ev[3], b->stream));
+    const int wr = batch_wait(b, b->ev[3]);
+    if (wr) return wr;
+    (void)collect_status(b, active.data(), A, status.data(), nullptr);
+    std::vector<int32_t> next;
+    for (int32_t i = 0; i < A; ++i) {
+      const int32_t k = active[i];
+      ++iters[k];
+      if (status[k] != RCM_OK) continue;
+      const int32_t w = std::min<int32_t>(32, b->n[k] - 1);
+      const int32_t* nodes = b->h_tail + (size_t)i * 96;
+      const int32_t* dists = nodes + 32;
+      const int32_t* degs = nodes + 64;
+      if ((uint32_t)dists[w - 1] == lastdist[k]) continue;  // converged: cur[k] is the start
+      int32_t idx = 0;  // first index holding the maximum distance
+      for (int32_t t = 1; t < w; ++t)
+        if (dists[t] > dists[idx]) idx = t;
+      int64_t minval = INT64_MAX;
+      for (int32_t t = idx; t < w; ++t) {  // minimum degree, later nodes win ties
+        if ((int64_t)degs[t] <= minval) {
+          minval = degs[t];
+          cur[k] = nodes[t];
+        }
+      }
+      lastdist[k] = (uint32_t)dists[w - 1];
+      next.push_back(k);
+    }
+    active.swap(next);
+  }
+  int rc = RCM_OK;
+  for (int32_t k = 0; k < C; ++k) {
+    start_out[k] = cur[k];
+    if (iterations_out) iterations_out[k] = iters[k];
+    if (status_out) status_out[k] = status[k];
+    if (status[k] && rc == RCM_OK) rc = status[k];
+  }
+  b->timing.start_sweeps = sweeps;
+  b->timing.start_search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* status_out) {
+  if (!b || !start) return RCM_EINVAL;
+  if (b->count == 0) return RCM_ENOTLOADED;
+  for (int32_t k = 0; k < b->count; ++k)
+    if (start[k] < 0 || start[k] >= b->n[k]) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(b->device));
+  b->have_perm = false;
+  const int32_t C = b->count;
+  const bool rev = (flags & RCM_FLAG_REVERSE) != 0;
+  RCM_HIP(hipMemcpyAsync(b->d_starts, start, (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipEventRecord(b->ev[0], b->stream));
+  auto enqueue_run = [&]() {
+    enqueue_many(b, b->d_order, C, false);
+    if (rev)
+      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)C * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
+                         b->d_order, C, kPerJobBlocks);
+  };
+  if (b->use_graph) {
+    hipGraphExec_t& ex = b->gexec[rev ? 1 : 0];
+    if (!ex) {
+      hipGraph_t& gr = b->graph[rev ? 1 : 0];
+      RCM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_run();
+      RCM_HIP(hipStreamEndCapture(b->stream, &gr));
+      RCM_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    } else {
+      ++b->timing.graph_replays;
+    }
+    RCM_HIP(hipGraphLaunch(ex, b->stream));
+  } else {
+    enqueue_run();
+  }
+  RCM_HIP(hipEventRecord(b->ev[1], b->stream));
+  RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipEventRecord(b->ev[3], b->stream));
+  const int wr = batch_wait(b, b->ev[3]);
+  if (wr) return wr;
+  float ms = 0.f;
+  RCM_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->timing.run_ms = ms;
+  b->timing.workers = batch_workers(b);
+  int64_t nb = 0;
+  std::vector<int32_t> st((size_t)C, RCM_OK);
+  const int rc = collect_status(b, b->order.data(), C, st.data(), &nb);
+  if (status_out) std::copy(st.begin(), st.end(), status_out);
+  b->ok_jobs.clear();  // matrices with a complete order (the bandwidth reads only these)
+  for (int32_t k : b->order)
+    if (st[k] == RCM_OK) b->ok_jobs.push_back(k);
+  b->timing.batches = nb;
+  b->reversed = rev;
+  b->have_perm = true;  // per-matrix status tells which orders are complete
+  return rc;
+}
+
+int rcm_batch_bandwidth(rcm_batch* b, int64_t* bw_out) {
+  if (!b || !bw_out) return RCM_EINVAL;
+  if (!b->have_perm) return RCM_ENOTLOADED;
+  RCM_HIP(hipSetDevice(b->device));
+  const int32_t C = b->count;
+  // only complete orders are permutations of [0, n): the inverse of any other would write out of range
+  const int32_t A = (int32_t)b->ok_jobs.size();
+  uint32_t* bw = b->scratch + 2 * (size_t)C;
+  RCM_HIP(hipMemsetAsync(bw, 0, (size_t)C * sizeof(uint32_t), b->stream));
+  RCM_HIP(hipEventRecord(b->ev[0], b->stream));
+  if (A > 0) {
+    RCM_HIP(hipMemcpyAsync(b->d_active, b->ok_jobs.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice,
+                           b->stream));
+    hipLaunchKernelGGL(k_inverse_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
+                       b->d_active, A, kPerJobBlocks);
+    hipLaunchKernelGGL(k_bandwidth_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
+                       b->d_active, A, kPerJobBlocks, bw);
+    if (b->reversed)  // aux held the reversed order: restore it (the bandwidth is the same either way)
+      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
+                         b->d_active, A, kPerJobBlocks);
+  }
+  RCM_HIP(hipEventRecord(b->ev[1], b->stream));
+  std::vector<uint32_t> h((size_t)C);
+  RCM_HIP(hipMemcpyAsync(h.data(), bw, (size_t)C * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  RCM_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->timing.bandwidth_ms = ms;
+  std::vector<uint8_t> ok((size_t)C, 0);
+  for (int32_t k : b->ok_jobs) ok[k] = 1;
+  for (int32_t k = 0; k < C; ++k) bw_out[k] = ok[k] ? (int64_t)h[k] : -1;
+  return RCM_OK;
+}
+
+int rcm_batch_read_perm(rcm_batch* b, const int64_t* po_off, int32_t* perm_cat) {
+  if (!b || !perm_cat) return RCM_EINVAL;
+  if (!b->have_perm) return RCM_ENOTLOADED;
+  RCM_HIP(hipSetDevice(b->device));
+  const int32_t* src = b->reversed ? b->aux : b->perm;
+  bool packed = true;
+  if (po_off)
+    for (int32_t k = 0; k < b->count && packed; ++k) packed = po_off[k] == b->pn[k];
+  if (packed) {
+    RCM_HIP(hipMemcpyAsync(perm_cat, src, (size_t)b->total_n * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+  } else {
+    for (int32_t k = 0; k < b->count; ++k)
+      RCM_HIP(hipMemcpyAsync(perm_cat + po_off[k], src + b->pn[k], (size_t)b->n[k] * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, b->stream));
+  }
+  RCM_HIP(hipStreamSynchronize(b->stream));
+  return RCM_OK;
+}
+
+int rcm_batch_perm_device(rcm_batch* b, int32_t k, const int32_t** d_perm_out) {
+  if (!b || !d_perm_out || k < 0 || k >= b->count) return RCM_EINVAL;
+  if (!b->have_perm) return RCM_ENOTLOADED;
+  *d_perm_out = (b->reversed ? b->aux : b->perm) + b->pn[k];
+  return RCM_OK;
+}
+
+int rcm_batch_timing(rcm_batch* b, rcm_batch_timing_t* out) {
+  if (!b || !out) return RCM_EINVAL;
+  *out = b->timing;
+  return RCM_OK;
+}
+
 int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, const int64_t* rp_off,
                        const int64_t* ci_off, const int32_t* row_ptr_cat, const int32_t* col_idx_cat,
                        const int32_t* start, uint32_t flags, const int64_t* po_off, int32_t* perm_cat,
@@ -515,23 +987,48 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
   if (count < 0 || (count > 0 && (!n || !nnz || !rp_off || !ci_off || !row_ptr_cat || !col_idx_cat || !po_off ||
                                   !perm_cat)))
     return RCM_EINVAL;
+  if (count == 0) return RCM_OK;
   int dev = 0;
   RCM_HIP(hipGetDevice(&dev));
-  rcm_ctx* c = nullptr;
-  int rc = rcm_ctx_create(dev, &c);
+  rcm_batch* b = nullptr;
+  int rc = rcm_batch_create(dev, &b);
   if (rc) return rc;
-  int worst = RCM_OK;
+  rcm_batch_set_param(b, RCM_PARAM_GRAPH, 0);  // one run: nothing to replay
+  std::vector<int32_t> st((size_t)count, 0), status((size_t)count, RCM_OK);
+  rc = rcm_batch_load(b, count, n, nnz, rp_off, ci_off, row_ptr_cat, col_idx_cat, status.data());
+  bool need_search = false;
   for (int32_t k = 0; k < count; ++k) {
-    int r = rcm_ctx_load(c, n[k], nnz[k], row_ptr_cat + rp_off[k], col_idx_cat + ci_off[k]);
-    int32_t s = start ? start[k] : -1;
-    if (!r && s < 0) r = rcm_ctx_find_start(c, &s, nullptr);
-    if (!r) r = rcm_ctx_run(c, s, flags);
-    if (!r && bandwidth_out && (flags & RCM_FLAG_BANDWIDTH)) r = rcm_ctx_bandwidth(c, bandwidth_out + k);
-    if (!r) r = rcm_ctx_read_perm(c, perm_cat + po_off[k]);
-    if (status_out) status_out[k] = r;
-    if (r && worst == RCM_OK) worst = r;
+    if (start && start[k] >= n[k]) {
+      status[k] = RCM_EINVAL;
+      if (!rc) rc = RCM_EINVAL;
+    }
+    if (!start || start[k] < 0) need_search = true;
   }
-  rcm_ctx_destroy(c);
+  // a matrix that fails (e.g. disconnected) keeps its status; the others are still reordered
+  std::vector<int32_t> st_search((size_t)count, RCM_OK), st_run((size_t)count, RCM_OK);
+  int hard = rc;  // launch-level failure (malformed input, device error, time-out)
+  if (!hard && need_search) {
+    const int r = rcm_batch_find_start(b, st.data(), nullptr, st_search.data());
+    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT) hard = r;
+  }
+  int worst = hard;
+  if (!hard) {
+    if (start)
+      for (int32_t k = 0; k < count; ++k)
+        if (start[k] >= 0) st[k] = start[k];
+    const int r = rcm_batch_run(b, st.data(), flags, st_run.data());
+    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT) hard = r;
+    for (int32_t k = 0; k < count; ++k) {
+      status[k] = (start && start[k] >= 0) ? st_run[k] : (st_search[k] ? st_search[k] : st_run[k]);
+      if (status[k] && worst == RCM_OK) worst = status[k];
+    }
+  }
+  if (!hard && bandwidth_out && (flags & RCM_FLAG_BANDWIDTH)) hard = rcm_batch_bandwidth(b, bandwidth_out);
+  if (!hard) hard = rcm_batch_read_perm(b, po_off, perm_cat);
+  if (hard && worst == RCM_OK) worst = hard;
+  if (status_out)
+    for (int32_t k = 0; k < count; ++k) status_out[k] = hard && status[k] == RCM_OK ? hard : status[k];
+  rcm_batch_destroy(b);
   return worst;
 }
 

@@ -55,12 +55,15 @@ struct Wave {
   }
 };
 
+struct LaneCtx;
 struct Block {
   std::vector<unsigned char> lds;
+  std::vector<LaneCtx*> lanes;  // diagnostics: every thread's context
   int nthreads = 64;
   std::atomic<int> arrived{0};
   std::atomic<int> phase{0};
   double wait_limit_s = 30.0;
+  void dump();
   // s_barrier of the workgroup: every thread (all waves) meets here
   void barrier() {
     const int ph = phase.load(std::memory_order_acquire);
@@ -76,6 +79,7 @@ struct Block {
       if ((spins & 1023) == 0 &&
           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > wait_limit_s) {
         std::fprintf(stderr, "wave_emu: a workgroup barrier was not reached by every thread\n");
+        dump();
         std::abort();
       }
     }
@@ -87,8 +91,22 @@ struct LaneCtx {
   Block* block = nullptr;
   Wave* wave = nullptr;
   int lane = 0;
+  // diagnostics: barriers passed and where the thread is (0 running, 1 workgroup barrier, 2 wave op, 3 done)
+  std::atomic<long> nblock{0}, nwave{0};
+  std::atomic<int> where{0};
 };
 inline thread_local LaneCtx ctx;
+
+inline void Block::dump() {
+  static std::atomic<int> once{0};
+  if (once.exchange(1)) return;
+  for (size_t t = 0; t < lanes.size(); ++t) {
+    LaneCtx* c = lanes[t];
+    if (!c) continue;
+    std::fprintf(stderr, "  thread %zu: where %d, workgroup barriers %ld, wave ops %ld\n", t, c->where.load(),
+                 c->nblock.load(), c->nwave.load());
+  }
+}
 
 template <class T>
 T& block_shared() {
@@ -103,11 +121,14 @@ T& block_shared() {
 // deposit v, return the slot values of all lanes via f
 template <class F>
 inline void exchange(uint64_t v, F&& f) {
+  ctx.where = 2;
+  ctx.nwave++;
   Wave& w = *ctx.wave;
   w.slot[ctx.lane] = v;
   w.barrier();
   f(w.slot);
   w.barrier();
+  ctx.where = 0;
 }
 
 // LDS bytes per emulated workgroup; the driver sets it to the kernel's shared struct size.
@@ -124,6 +145,7 @@ void launch(unsigned grid, unsigned block, bool concurrent, K kernel, A... args)
     for (auto& bk : blocks) {
       bk.lds.assign(emu_lds_bytes, 0xA5);  // device LDS is not zeroed either
       bk.nthreads = (int)block;
+      bk.lanes.assign(block, nullptr);
     }
     std::vector<std::thread> th;
     for (unsigned b = b0; b < b1; ++b)
@@ -136,7 +158,9 @@ void launch(unsigned grid, unsigned block, bool concurrent, K kernel, A... args)
           ctx.block = &blocks[b - b0];
           ctx.wave = &waves[(size_t)(b - b0) * nwaves + t / 64];
           ctx.lane = (int)(t % 64);
+          blocks[b - b0].lanes[t] = &ctx;
           kernel(args...);
+          ctx.where = 3;
         });
     for (auto& x : th) x.join();
   };
@@ -158,7 +182,10 @@ inline int __lane_id() { return emu::ctx.lane; }
 // workgroup barrier with a workgroup-scope fence, as __syncthreads() on the device
 inline void __syncthreads() {
   std::atomic_thread_fence(std::memory_order_seq_cst);
+  emu::ctx.where = 1;
+  emu::ctx.nblock++;
   emu::ctx.block->barrier();
+  emu::ctx.where = 0;
 }
 
 inline unsigned long long __ballot(int pred) {
@@ -176,10 +203,21 @@ inline int __shfl_up(int v, unsigned d, int width = 64) {
   });
   return r;
 }
+inline int __shfl(int v, int src, int width = 64) {
+  int r = v;
+  emu::exchange((uint32_t)v, [&](const uint64_t* s) { r = (int)(uint32_t)s[src % 64]; });
+  (void)width;
+  return r;
+}
 inline int __shfl_xor(int v, int m, int width = 64) {
   int r = v;
   emu::exchange((uint32_t)v, [&](const uint64_t* s) { r = (int)(uint32_t)s[(emu::ctx.lane ^ m) % 64]; });
   (void)width;
+  return r;
+}
+inline int __builtin_amdgcn_readfirstlane(int v) {
+  int r = 0;
+  emu::exchange((uint32_t)v, [&](const uint64_t* s) { r = (int)(uint32_t)s[0]; });
   return r;
 }
 inline int __builtin_amdgcn_readlane(int v, int lane) {
@@ -245,10 +283,13 @@ inline T __hip_atomic_fetch_min(T* p, V v, int, int) {
   return cur;
 }
 template <class T, class V>
+inline T __hip_atomic_exchange(T* p, V v, int, int) { return __atomic_exchange_n(p, (T)v, __ATOMIC_SEQ_CST); }
+template <class T, class V>
 inline bool __hip_atomic_compare_exchange_strong(T* p, T* expected, V desired, int, int, int) {
   return __atomic_compare_exchange_n(p, expected, (T)desired, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
 }
 #endif
+inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned atomicMax(unsigned* p, unsigned v) {
   unsigned cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);

@@ -90,3 +90,19 @@ def shuffle_rows(rp, ci, seed):
         rng.shuffle(seg)
         ci[rp[i]:rp[i + 1]] = seg
     return rp, ci
+
+
+def hub_graph(leaves, extra, seed, hubs=1, diag=True):
+    """`hubs` nodes adjacent to every one of `leaves` other nodes (rows longer than a batch's slots), plus
+    `extra` random edges among the leaves so that the leaves' degrees vary; node ids shuffled."""
+    rng = np.random.default_rng(seed)
+    n = hubs + leaves
+    e = [(h, hubs + i) for h in range(hubs) for i in range(leaves)]
+    e += [(h, h + 1) for h in range(hubs - 1)]
+    for _ in range(extra):
+        a, b = rng.integers(hubs, n, 2)
+        if a != b:
+            e.append((int(a), int(b)))
+    perm = rng.permutation(n)
+    e = [(int(perm[a]), int(perm[b])) for a, b in e]
+    return from_edges(n, e, diag)

@@ -160,3 +160,24 @@ def test_batch_mix_layout():
         assert O.cm(m.row_offsets, m.col_ids, 0).shape[0] == m.rows  # connected after repair
     bs2 = P.workloads.batch_mix(6, 200, 3000, seed=4, threads=1)
     np.testing.assert_array_equal(bs.col_idx, bs2.col_idx)  # independent of the thread count
+
+
+def test_batch_create_without_gpu_fails_cleanly():
+    import ctypes as C
+
+    h = C.c_void_p()
+    rc = _native.lib().rcm_batch_create(0, C.byref(h))
+    if rc == 0:  # a GPU is visible: nothing to check here
+        _native.lib().rcm_batch_destroy(h)
+    else:
+        assert rc == P.RCM_EDEVICE
+
+
+def test_batchset_from_matrices_round_trip():
+    mats = [P.workloads.stencil2d(5, 4), P.CSR(*G.path(7)), P.workloads.stencil3d(3, 3, 2)]
+    bs = P.BatchSet.from_matrices(mats)
+    assert len(bs) == 3
+    for k, m in enumerate(mats):
+        got = bs.matrix(k)
+        assert np.array_equal(got.row_offsets, m.row_offsets) and np.array_equal(got.col_ids, m.col_ids)
+    assert bs.perm_offsets.tolist() == [0, 20, 27]

@@ -73,3 +73,82 @@ def test_disconnected_places_component_only():
     perm, _, info = emu_lib.run(rp, ci, 0, workers=2, engine="block")
     assert info[0] == 0 and info[1] == 3
     assert perm[:3].tolist() == [0, 1, 2]
+
+
+def _many_set():
+    mats = [CASES[k] for k in ("path17", "star40", "complete12", "grid9x7", "random300", "duplicates300",
+                               "stencil24x20")]
+    mats += [G.random_connected(80 + 40 * i, 120 + 30 * i, seed=20 + i) for i in range(5)]
+    mats += [G.path(1), G.path(2)]
+    return mats
+
+
+@pytest.mark.parametrize("workers", [1, 3, 7])
+def test_many_matrices_one_launch(workers):
+    mats = _many_set()
+    starts = [(len(rp) - 1) // 3 for rp, _ in mats]
+    perms, _, info, done = emu_lib.run_many(mats, starts, workers=workers)
+    assert done == len(mats)
+    for k, ((rp, ci), s) in enumerate(zip(mats, starts)):
+        assert info[k, 0] == 0 and info[k, 3] == 1, (k, info[k].tolist())
+        assert info[k, 1] == rp.shape[0] - 1
+        assert np.array_equal(perms[k], O.cm(rp, ci, s)), k
+
+
+def test_many_matrices_bfs_levels():
+    mats = _many_set()
+    starts = [0 for _ in mats]
+    perms, levs, info, done = emu_lib.run_many(mats, starts, workers=4, bfs=True)
+    assert done == len(mats)
+    for k, (rp, ci) in enumerate(mats):
+        order, level = O.bfs(rp, ci, 0)
+        assert np.array_equal(perms[k], order), k
+        assert np.array_equal(levs[k], level), k
+
+
+def test_many_matrices_disconnected_member():
+    rp, ci = G.from_edges(10, [(0, 1), (1, 2), (5, 6)])
+    mats = [G.path(9), (rp, ci), G.star(12)]
+    perms, _, info, done = emu_lib.run_many(mats, [0, 0, 0], workers=2)
+    assert done == 3
+    assert info[1, 0] == 0 and info[1, 1] == 3
+    assert np.array_equal(perms[0], O.cm(*mats[0], 0)) and np.array_equal(perms[2], O.cm(*mats[2], 0))
+
+
+LONG = {
+    "star3000": lambda: G.star(3000),
+    "hub5000": lambda: G.hub_graph(5000, 9000, seed=1),
+    "twohubs2600": lambda: G.hub_graph(2600, 3000, seed=2, hubs=2),
+    "hubdups": lambda: G.with_duplicates(*G.hub_graph(2500, 2000, seed=3), seed=4),
+}
+
+
+@pytest.mark.parametrize("name", list(LONG))
+@pytest.mark.parametrize("workers", [1, 3])
+def test_long_rows(name, workers):
+    rp, ci = LONG[name]()
+    n = rp.shape[0] - 1
+    deg = np.diff(rp)
+    hub = int(np.argmax(deg))
+    assert deg[hub] > 2048
+    for start in sorted({hub, int(np.argmin(deg)), n // 2}):
+        perm, _, info = emu_lib.run(rp, ci, start, workers=workers, engine="block", timeout_s=60)
+        assert info[0] == 0, f"device error {info.tolist()}"
+        assert np.array_equal(perm, O.cm(rp, ci, start)), (name, start)
+
+
+def test_long_rows_bfs_levels():
+    rp, ci = LONG["twohubs2600"]()
+    perm, lev, info = emu_lib.run(rp, ci, 5, workers=2, bfs=True, engine="block", timeout_s=60)
+    assert info[0] == 0
+    order, level = O.bfs(rp, ci, 5)
+    assert np.array_equal(perm, order) and np.array_equal(lev, level)
+
+
+def test_long_rows_many():
+    mats = [LONG["hub5000"](), G.path(30), LONG["star3000"]()]
+    starts = [0, 0, 0]
+    perms, _, info, done = emu_lib.run_many(mats, starts, workers=3, timeout_s=60)
+    assert done == 3
+    for k, (rp, ci) in enumerate(mats):
+        assert info[k, 0] == 0 and np.array_equal(perms[k], O.cm(rp, ci, 0)), k

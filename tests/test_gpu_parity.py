@@ -174,3 +174,39 @@ def test_context_reuse_across_sizes(ctx):
             np.testing.assert_array_equal(ctx.perm(), ref)
     finally:
         ctx.set_param("workers", 0)
+
+
+def _rmat(scale, seed):
+    m = P.workloads.rmat(scale, seed=seed)
+    m, _ = P.connect_components(m)
+    return m
+
+
+LONG = {
+    "star3000": lambda: P.CSR(*G.star(3000)),
+    "hub5000": lambda: P.CSR(*G.hub_graph(5000, 9000, seed=1)),
+    "twohubs2600": lambda: P.CSR(*G.hub_graph(2600, 3000, seed=2, hubs=2)),
+    "hubdups": lambda: P.CSR(*G.with_duplicates(*G.hub_graph(2500, 2000, seed=3), seed=4)),
+    "rmat14": lambda: _rmat(14, 5),
+    "rmat16": lambda: _rmat(16, 6),
+}
+
+
+@pytest.mark.parametrize("name", list(LONG))
+def test_long_rows_match_oracle(ctx, name):
+    """Rows longer than a batch's 2048 adjacency slots (hubs): the long-row path, bit-exact."""
+    m = LONG[name]()
+    deg = np.diff(m.row_offsets)
+    assert deg.max() > 2048
+    rc, ps, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+    assert rc == 0
+    for start in sorted({int(np.argmax(deg)), int(np.argmin(deg)), ps}):
+        _say("long rows", name, "start", start)
+        ctx.load(m)
+        ctx.run(start, P.RCM_FLAG_STABLE)
+        assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, start)), (name, start)
+    _say("long rows start search", name)
+    assert ctx.find_start() == ps
+    ctx.run(ps)
+    want = O.cm(m.row_offsets, m.col_ids, ps)
+    assert ctx.bandwidth() == O.bandwidth(m.row_offsets, m.col_ids, want)
