@@ -7,10 +7,13 @@ seeded symmetric random relabelling, resident in HBM. A step is one Cuthill-McKe
 once on the GPU before timing (the reference also excludes the start search from its timed region,
 src/device/rcm_gpu_batch.cu:47-99).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched COUNT] [--serving SECONDS]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batched COUNT] [--serving REQUESTS]
 
 Fields beyond the contract: kernel-only nnz/s, end-to-end time (upload + start search + reorder + bandwidth
-+ download), the bandwidth of the result, and, when asked for, the batched and serving forms.
++ download), the bandwidth of the result, the batched form (4096-matrix mix, n log-uniform in [1k, 200k],
+stencils and random geometric graphs: matrices/s of one persistent launch against one-at-a-time plain GPU
+calls on a seeded sample of the same matrices) and the serving form (a seeded Poisson stream of requests,
+95% that mix and 5% R-MAT scale 18-20, at several arrival rates: p50/p99 latency and throughput).
 """
 from __future__ import annotations
 
@@ -72,6 +75,132 @@ print("CPUBASE " + json.dumps(out))
     return None
 
 
+def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int):
+    """Batched form on the 4096-matrix mix: every matrix reordered by one persistent launch (device-resident,
+    hipGraph replay), against one-at-a-time plain GPU calls (rcm_ctx_run per matrix, inputs resident) on a
+    seeded sample of the same matrices."""
+    import numpy as np
+
+    import parallelbatchrcm_amd as P
+
+    t0 = time.perf_counter()
+    bs = P.workloads.batch_mix(count, seed=seed)
+    gen_s = time.perf_counter() - t0
+    out = {"matrices": count, "sum_n": int(bs.n.astype(np.int64).sum()), "sum_nnz": int(bs.nnz.sum()),
+           "generate_s": gen_s}
+    e2e0 = time.perf_counter()
+    with P.RcmBatch(timeout_ms=120000, graph=True) as b:
+        b.load(bs)
+        starts, iters, st = b.find_start()
+        if not (st == 0).all():
+            raise RuntimeError(f"batched start search failed for {int((st != 0).sum())} matrices")
+        st = b.run(starts)
+        bw = b.bandwidth()
+        perms = b.perms()
+        e2e_s = time.perf_counter() - e2e0
+        if not (st == 0).all():
+            raise RuntimeError("batched run failed")
+        tm = b.timing()
+        out.update(load_ms=tm.load_ms, start_search_ms=tm.start_search_ms, start_sweeps=tm.start_sweeps,
+                   end_to_end_s=e2e_s, workers=tm.workers)
+        for _ in range(warmup):
+            b.run(starts)
+        walls, devs = [], []
+        for _ in range(steps):
+            w0 = time.perf_counter()
+            b.run(starts)
+            walls.append(time.perf_counter() - w0)
+            devs.append(b.timing().run_ms)
+        tm = b.timing()
+        out.update(run_ms_mean=float(np.mean(devs)), run_wall_ms_mean=float(np.mean(walls)) * 1e3,
+                   matrices_per_s=count / float(np.mean(walls)), nnz_per_s=out["sum_nnz"] / float(np.mean(walls)),
+                   batches=int(tm.batches), graph_replays=int(tm.graph_replays))
+        # sanity: every order is a permutation of its matrix
+        for k in range(0, count, max(1, count // 64)):
+            pk = perms[k]
+            if pk.shape[0] != bs.n[k] or np.bincount(pk, minlength=pk.shape[0]).max() != 1:
+                raise RuntimeError(f"matrix {k}: not a permutation")
+    # one-at-a-time plain calls on a seeded sample (inputs resident, start known: the same work per matrix)
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(count, size=min(sample, count), replace=False))
+    single = 0.0
+    with P.RcmContext(timeout_ms=60000) as c:
+        c.load(bs.matrix(int(idx[0])))
+        c.run(int(starts[idx[0]]))  # warm the kernel
+        for k in idx:
+            c.load(bs.matrix(int(k)))
+            w0 = time.perf_counter()
+            c.run(int(starts[k]))
+            single += time.perf_counter() - w0
+    out["one_at_a_time_sample"] = int(idx.shape[0])
+    out["one_at_a_time_matrices_per_s"] = idx.shape[0] / single
+    out["speedup_vs_one_at_a_time"] = out["matrices_per_s"] / out["one_at_a_time_matrices_per_s"]
+    out["bandwidth_sum"] = int(bw.sum())
+    return out
+
+
+def bench_serving(requests: int, rates, seed: int, lanes: int):
+    """Serving form: a seeded Poisson stream of reorder requests (95% the batched mix, 5% R-MAT scale 18-20,
+    a,b,c = .57,.19,.19, symmetrised, diagonal, connected) through rcm_server; latency is arrival -> result
+    on the host. One run per arrival rate."""
+    import threading
+
+    import numpy as np
+
+    import parallelbatchrcm_amd as P
+
+    rng = np.random.default_rng(seed)
+    nmix = requests - max(1, requests // 20)
+    bs = P.workloads.batch_mix(nmix, seed=seed)
+    pool = [bs.matrix(k) for k in range(nmix)]
+    rmats = [P.connect_components(P.workloads.rmat(sc, 16, 0.57, 0.19, 0.19, seed=seed + sc))[0] for sc in (18, 19, 20)]
+    reqs = pool + [rmats[i % 3] for i in range(requests - nmix)]
+    order = rng.permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    out = {"requests": len(reqs), "rmat_requests": requests - nmix, "lanes": lanes, "runs": []}
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, size=len(reqs))
+        lat, dev, que = [], [], []
+        with P.RcmServer(lanes=lanes) as srv:
+            ids = [None] * len(reqs)
+            done = threading.Event()
+
+            def collect():
+                for i in range(len(reqs)):
+                    while ids[i] is None:
+                        time.sleep(0.0002)
+                    rc, perm, js = srv.wait(ids[i], 120000)
+                    if rc != 0 or perm.shape[0] != reqs[i].rows:
+                        raise RuntimeError(f"request {i} failed: {rc}")
+                    lat.append(js.latency_ms)
+                    dev.append(js.device_ms)
+                    que.append(js.queue_ms)
+                done.set()
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t0 = time.perf_counter()
+            due = t0
+            for i, m in enumerate(reqs):
+                due += gaps[i]
+                dt = due - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(m, -1, P.RCM_FLAG_STABLE)
+            th.join()
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        out["runs"].append({
+            "arrival_rate_per_s": rate, "throughput_per_s": len(reqs) / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(np.mean(lat)), "p99_queue_ms": float(np.percentile(que, 99)),
+            "p50_device_ms": float(np.percentile(dev, 50)), "max_inflight": int(st.max_inflight),
+            "groups": int(st.loop_iterations)})
+        print(f"[bench] serving rate {rate}/s: p50 {out['runs'][-1]['p50_ms']:.1f} ms "
+              f"p99 {out['runs'][-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,8 +208,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched", type=int, default=0, help="also run the batched form on COUNT matrices")
-    ap.add_argument("--serving", type=float, default=0.0, help="also run the serving form for SECONDS")
+    ap.add_argument("--batched", type=int, default=4096,
+                    help="batched form on COUNT matrices of the mix (0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--one-at-a-time-sample", type=int, default=256)
+    ap.add_argument("--serving", type=int, default=400, help="serving form: requests per arrival rate (0 = skip)")
+    ap.add_argument("--serving-rates", type=str, default="200,1000,4000")
+    ap.add_argument("--serving-lanes", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -154,6 +288,15 @@ def main() -> int:
         "generate_s": gen_s,
     }
     ctx.close()
+
+    if args.batched > 0:
+        print(f"[bench] batched form: {args.batched} matrices", file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(args.batched, 4, args.batched_steps, 1, args.one_at_a_time_sample)
+
+    if args.serving > 0:
+        rates = [float(x) for x in args.serving_rates.split(",") if x]
+        print(f"[bench] serving form: {args.serving} requests at {rates} /s", file=sys.stderr, flush=True)
+        line["serving"] = bench_serving(args.serving, rates, 5, args.serving_lanes)
 
     if not args.no_cpu:
         cb = cpu_baseline()

@@ -13,6 +13,7 @@
  *                                 can keep the CSR resident in HBM between calls
  *   rcm_batch_* / rcm_gpu_batch_many  no single reference counterpart: GPU_BATCH::rcm applied to many
  *                                 matrices, done in one persistent launch
+ *   rcm_server_*                  no reference counterpart: continuous batching of reorder requests
  *
  * Conventions
  *   - CSR is int32: row_ptr[n+1], col_idx[nnz]; 1 <= n < 2^30, nnz < 2^31. Values are not needed.
@@ -46,7 +47,10 @@ enum rcm_status {
   RCM_EDEVICE = -4,        /* HIP runtime error */
   RCM_ETIMEOUT = -5,       /* device watchdog expired (see RCM_PARAM_TIMEOUT_MS) */
   RCM_ECAPACITY = -6,      /* an internal capacity was exceeded */
-  RCM_ENOTLOADED = -7      /* context has no matrix loaded */
+  RCM_ENOTLOADED = -7,     /* context has no matrix loaded */
+  RCM_ECANCELED = -8,      /* serving: the server shut down before the request finished */
+  RCM_ENOTFOUND = -9,      /* serving: unknown request id */
+  RCM_EAGAIN = -10         /* serving: the request has not finished within the wait's time limit */
 };
 
 /* One-shot call on host buffers: upload, (optional GPU start search), reorder, (optional bandwidth),
@@ -162,6 +166,60 @@ int rcm_batch_read_perm(rcm_batch* b, const int64_t* po_off, int32_t* perm_cat);
 /* Device pointer to matrix k's order (n[k] entries), same orientation as rcm_batch_read_perm. */
 int rcm_batch_perm_device(rcm_batch* b, int32_t k, const int32_t** d_perm_out);
 int rcm_batch_timing(rcm_batch* b, rcm_batch_timing_t* out);
+
+/* ---- serving: continuous batching of reorder requests on one GPU ---- */
+/* No reference counterpart (the reference reorders one matrix per process run, src/main.cpp:503-515).
+ * The server runs `lanes` many-matrix contexts side by side, each with its own stream, its share of the
+ * persistent workgroups and a fixed arena of HBM allocated at creation. A free lane takes every waiting
+ * request that fits its arena (first come first served, looking 8 ahead; at most `slots` per group) and
+ * reorders them in one persistent launch (start-search sweeps for requests without a start, then the run);
+ * requests that arrive while others are on the GPU go to the next free lane and overlap with them on the
+ * device. Every result is bit-identical to rcm_gpu_batch on the same matrix, alone or in any mix. */
+typedef struct rcm_server rcm_server;
+
+typedef struct rcm_server_config {
+  int32_t slots;           /* requests per group at most (1..4096); 0 = 256 */
+  int32_t lanes;           /* groups on the GPU at once (1..16); 0 = 4 */
+  int64_t arena_bytes;     /* device memory for all lanes; 0 = half of the free HBM, at most 64 GiB */
+  int32_t workers;         /* persistent workgroups per lane; 0 = the co-resident capacity / lanes */
+  int64_t job_timeout_ms;  /* device watchdog of a launch; 0 = 60000 */
+} rcm_server_config;
+
+typedef struct rcm_job_stats {
+  int32_t status;      /* rcm_status of the request */
+  int32_t start_node;  /* start used */
+  int32_t sweeps;      /* start-search sweeps (0 when a start was given) */
+  int32_t lane;        /* lane the request ran on (-1 if never admitted) */
+  double queue_ms;     /* submit -> admitted to a lane */
+  double device_ms;    /* admitted -> result on the host */
+  double latency_ms;   /* submit -> result on the host */
+  int64_t bandwidth;   /* with RCM_FLAG_BANDWIDTH, else -1 */
+} rcm_job_stats;
+
+typedef struct rcm_server_stats {
+  int64_t submitted, completed, failed;
+  int32_t inflight, pending, max_inflight, slots;
+  int64_t arena_bytes, arena_peak_bytes;  /* all lanes; peak of the bytes admitted at once */
+  int32_t workers;                        /* persistent workgroups of all lanes */
+  int32_t kernel_running;                 /* always 0: no kernel outlives a group */
+  int64_t admitted;                       /* requests admitted to a lane */
+  int64_t loop_iterations;                /* groups formed */
+  int64_t probe_rounds;                   /* unused (0) */
+  int64_t claims;                         /* unused (0) */
+} rcm_server_stats;
+
+int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out);
+/* Queue a request (the CSR is copied; the caller's buffers may be reused at once). start_node < 0 picks the
+ * pseudo-peripheral start. Returns RCM_ENOMEM if the matrix can never fit a lane's arena. */
+int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                      int32_t start_node, uint32_t flags, int64_t* id_out);
+/* Wait for request `id` (timeout_ms < 0: no limit), copy its order (n entries, reversed with
+ * RCM_FLAG_REVERSE) into perm_out (nullable) and its statistics into stats (nullable), and release it.
+ * Returns the request's status, or RCM_EAGAIN if it has not finished in time (it stays queued). */
+int rcm_server_wait(rcm_server* s, int64_t id, int64_t timeout_ms, int32_t* perm_out, rcm_job_stats* stats);
+int rcm_server_stats_get(rcm_server* s, rcm_server_stats* out);
+/* Stop admitting, cancel what has not been admitted, let admitted groups finish, release the lanes. */
+int rcm_server_destroy(rcm_server* s);
 
 /* One-shot form over host buffers. Reorders `count` independent matrices stored back to back on the host:
  *   row_ptr_cat + rp_off[k] holds matrix k's n[k]+1 local offsets, col_idx_cat + ci_off[k] its nnz[k] columns.

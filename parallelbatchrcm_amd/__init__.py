@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native
-from ._native import I32P, I64P, BatchTimingT, CsrT, TimingT
+from ._native import I32P, I64P, BatchTimingT, CsrT, JobStatsT, ServerConfigT, ServerStatsT, TimingT
 
 RCM_FLAG_REVERSE = 1
 RCM_FLAG_STABLE = 2
@@ -33,6 +33,9 @@ RCM_EDEVICE = -4
 RCM_ETIMEOUT = -5
 RCM_ECAPACITY = -6
 RCM_ENOTLOADED = -7
+RCM_ECANCELED = -8
+RCM_ENOTFOUND = -9
+RCM_EAGAIN = -10
 
 PARAM_WORKERS = 1
 PARAM_TIMEOUT_MS = 2
@@ -465,6 +468,61 @@ class RcmBatch:
     def close(self) -> None:
         if self._h:
             _native.lib().rcm_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RcmServer:
+    """Continuous batching of reorder requests on one GPU (rcm_server in include/rcm_gpu_batch_abi.h): lanes of
+    many-matrix contexts run side by side; a free lane takes every waiting request that fits its HBM arena and
+    reorders them in one persistent launch, so requests that arrive while others run overlap with them on the
+    device. Results are bit-identical to rcm_gpu_batch."""
+
+    def __init__(self, device: int = 0, slots: int = 256, lanes: int = 4, arena_bytes: int = 0, workers: int = 0,
+                 job_timeout_ms: int = 60000):
+        cfg = ServerConfigT(slots, lanes, arena_bytes, workers, job_timeout_ms)
+        h = C.c_void_p()
+        _check(_native.lib().rcm_server_create(device, C.byref(cfg), C.byref(h)), "rcm_server_create")
+        self._h = h
+        self._n = {}
+
+    def submit(self, m: CSR, start: int = -1, flags: int = RCM_FLAG_STABLE) -> int:
+        jid = C.c_int64()
+        _check(_native.lib().rcm_server_submit(self._h, m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids),
+                                               int(start), flags, C.byref(jid)), "rcm_server_submit")
+        self._n[jid.value] = m.rows
+        return jid.value
+
+    def wait(self, jid: int, timeout_ms: int = -1):
+        """Returns (status, perm or None, JobStatsT); raises RcmError(RCM_EAGAIN) on time-out."""
+        n = self._n[jid]
+        perm = np.empty(n, np.int32)
+        st = JobStatsT()
+        rc = _native.lib().rcm_server_wait(self._h, jid, int(timeout_ms), _p32(perm), C.byref(st))
+        if rc == RCM_EAGAIN:
+            raise RcmError(rc, "rcm_server_wait")
+        del self._n[jid]
+        return rc, (perm if rc == RCM_OK else None), st
+
+    def stats(self) -> ServerStatsT:
+        st = ServerStatsT()
+        _check(_native.lib().rcm_server_stats_get(self._h, C.byref(st)), "rcm_server_stats_get")
+        return st
+
+    def close(self) -> None:
+        if self._h:
+            _native.lib().rcm_server_destroy(self._h)
             self._h = None
 
     def __enter__(self):

@@ -52,6 +52,49 @@ class BatchTimingT(C.Structure):
     ]
 
 
+class ServerConfigT(C.Structure):
+    _fields_ = [
+        ("slots", C.c_int32),
+        ("lanes", C.c_int32),
+        ("arena_bytes", C.c_int64),
+        ("workers", C.c_int32),
+        ("job_timeout_ms", C.c_int64),
+    ]
+
+
+class JobStatsT(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("start_node", C.c_int32),
+        ("sweeps", C.c_int32),
+        ("lane", C.c_int32),
+        ("queue_ms", C.c_double),
+        ("device_ms", C.c_double),
+        ("latency_ms", C.c_double),
+        ("bandwidth", C.c_int64),
+    ]
+
+
+class ServerStatsT(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("failed", C.c_int64),
+        ("inflight", C.c_int32),
+        ("pending", C.c_int32),
+        ("max_inflight", C.c_int32),
+        ("slots", C.c_int32),
+        ("arena_bytes", C.c_int64),
+        ("arena_peak_bytes", C.c_int64),
+        ("workers", C.c_int32),
+        ("kernel_running", C.c_int32),
+        ("admitted", C.c_int64),
+        ("loop_iterations", C.c_int64),
+        ("probe_rounds", C.c_int64),
+        ("claims", C.c_int64),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     # rcm_gpu_batch_abi.h
@@ -80,6 +123,11 @@ _SIGNATURES = {
     "rcm_batch_read_perm": (C.c_int, [C.c_void_p, I64P, I32P]),
     "rcm_batch_perm_device": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
     "rcm_batch_timing": (C.c_int, [C.c_void_p, C.POINTER(BatchTimingT)]),
+    "rcm_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigT), C.POINTER(C.c_void_p)]),
+    "rcm_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_uint32, I64P]),
+    "rcm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, I32P, C.POINTER(JobStatsT)]),
+    "rcm_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsT)]),
+    "rcm_server_destroy": (C.c_int, [C.c_void_p]),
     "rcm_gpu_batch_many": (
         C.c_int,
         [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],

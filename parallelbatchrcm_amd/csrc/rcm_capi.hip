@@ -3,12 +3,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "engine_many.hpp"
@@ -228,7 +235,11 @@ struct rcm_batch {
   std::vector<int64_t> nnz, pn;  // pn[k] = sum_{i<k} n[i]
   std::vector<int32_t> order;    // all jobs, largest nnz first
   int64_t total_n = 0, total_nnz = 0;
-  // device buffers (one allocation set per load)
+  // device buffers: views into one arena (grown on demand, or fixed for a serving lane)
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  bool arena_fixed = false;
+  int32_t hcap = 0;  // matrices the pinned host mirrors hold
   int32_t *rp = nullptr, *ci = nullptr;
   int32_t *perm = nullptr, *level = nullptr, *aux = nullptr;
   uint64_t *qinfo = nullptr, *claim = nullptr, *bstat = nullptr, *brange = nullptr;
@@ -255,37 +266,87 @@ namespace {
 
 constexpr int kPerJobBlocks = 4;  // helper-kernel workgroups per matrix
 
-void batch_free_set(rcm_batch* b) {
+// Byte layout of a loaded set in the arena: every buffer 256-byte aligned.
+struct BatchLayout {
+  size_t rp, ci, perm, level, aux, qinfo, claim, bstat, brange, ctl, jobs, order, starts, active, tail, nnz,
+      scratch, g, total;
+};
+BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  BatchLayout L{};
+  size_t o = 0;
+  auto put = [&](size_t& f, size_t bytes) { f = o; o += al(std::max<size_t>(bytes, 1)); };
+  put(L.rp, 4 * rp_end);
+  put(L.ci, 4 * ci_end);
+  put(L.perm, 4 * N);
+  put(L.level, 4 * N);
+  put(L.aux, 4 * N);
+  put(L.qinfo, 8 * N);
+  put(L.claim, 8 * N);
+  put(L.bstat, 8 * (N + C));
+  put(L.brange, 8 * (N + C));
+  put(L.ctl, sizeof(Control) * C);
+  put(L.jobs, sizeof(JobDesc) * C);
+  put(L.order, 4 * C);
+  put(L.starts, 4 * C);
+  put(L.active, 4 * C);
+  put(L.tail, 4 * 96 * C);
+  put(L.nnz, 8 * C);
+  put(L.scratch, 4 * 3 * C);
+  put(L.g, sizeof(ManyControl));
+  L.total = o;
+  return L;
+}
+
+// Forget the loaded set (its captured graphs refer to its buffers); the memory is kept for the next load.
+void batch_reset(rcm_batch* b) {
   for (int r = 0; r < 2; ++r) {
     if (b->gexec[r]) (void)hipGraphExecDestroy(b->gexec[r]);
     if (b->graph[r]) (void)hipGraphDestroy(b->graph[r]);
     b->gexec[r] = nullptr;
     b->graph[r] = nullptr;
   }
-  dev_free(b->rp);
-  dev_free(b->ci);
-  dev_free(b->perm);
-  dev_free(b->level);
-  dev_free(b->aux);
-  dev_free(b->qinfo);
-  dev_free(b->claim);
-  dev_free(b->bstat);
-  dev_free(b->brange);
-  dev_free(b->ctl);
-  dev_free(b->jobs);
-  dev_free(b->d_order);
-  dev_free(b->d_starts);
-  dev_free(b->d_active);
-  dev_free(b->tail);
-  dev_free(b->d_nnz);
-  dev_free(b->scratch);
-  dev_free(b->g);
+  b->count = 0;
+  b->have_perm = false;
+}
+
+void batch_release(rcm_batch* b) {
+  batch_reset(b);
+  if (b->arena) (void)hipFree(b->arena);
+  b->arena = nullptr;
+  b->arena_cap = 0;
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   if (b->h_tail) (void)hipHostFree(b->h_tail);
   b->h_ctl = nullptr;
   b->h_tail = nullptr;
-  b->count = 0;
-  b->have_perm = false;
+  b->hcap = 0;
+}
+
+// Device arena of at least `bytes` (grown unless fixed) and pinned mirrors for `count` matrices.
+int batch_reserve(rcm_batch* b, size_t bytes, int32_t count) {
+  if (bytes > b->arena_cap) {
+    if (b->arena_fixed) return RCM_ENOMEM;
+    if (b->arena) (void)hipFree(b->arena);
+    b->arena = nullptr;
+    b->arena_cap = 0;
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(reinterpret_cast<void**>(&b->arena), want) != hipSuccess) return RCM_ENOMEM;
+    b->arena_cap = want;
+  }
+  if (count > b->hcap) {
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+    if (b->h_tail) (void)hipHostFree(b->h_tail);
+    b->h_ctl = nullptr;
+    b->h_tail = nullptr;
+    b->hcap = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&b->h_ctl), (size_t)count * sizeof(Control), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&b->h_tail), (size_t)count * 96 * sizeof(int32_t),
+                      hipHostMallocDefault) != hipSuccess)
+      return RCM_ENOMEM;
+    b->hcap = count;
+  }
+  return RCM_OK;
 }
 
 ManyArgs batch_args(rcm_batch* b, const int32_t* d_order, int32_t count) {
@@ -351,6 +412,195 @@ int batch_wait(rcm_batch* b, hipEvent_t ev) {
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------------------------------------
+// Serving: continuous batching over lanes. A lane is a many-matrix context (its own stream, a fixed share of
+// the persistent workgroups and a fixed arena of HBM) driven by its own host thread. A free lane takes
+// every waiting request that fits its arena (first come first served, looking 8 ahead), reorders them in
+// one persistent launch (start search sweeps, then the run) and delivers the results; the lanes run at the
+// same time, so requests that arrive while others are on the GPU start on the next free lane and overlap
+// with them on the device. Arenas are allocated once: no allocation or free happens while kernels run.
+// ---------------------------------------------------------------------------------------------------------
+namespace {
+
+struct ServerJob {
+  int64_t id = 0;
+  int32_t n = 0;
+  int64_t nnz = 0;
+  std::vector<int32_t> rp, ci;
+  int32_t start_req = -1;
+  uint32_t flags = 0;
+  size_t bytes = 0;  // arena bytes of this matrix
+  bool done = false;
+  int status = RCM_OK;
+  std::vector<int32_t> perm;
+  int64_t bw = -1;
+  int32_t start_used = -1, sweeps = 0, lane = -1;
+  std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
+};
+
+// Arena bytes of one matrix inside a lane's set (batch_layout without the alignment of shared buffers).
+size_t job_bytes(int32_t n, int64_t nnz) {
+  const size_t N = (size_t)n;
+  return 4 * (N + 1) + 4 * (size_t)nnz + 12 * N + 16 * N + 16 * (N + 1) + sizeof(Control) + sizeof(JobDesc) + 4 * 3 +
+         4 * 96 + 8 + 12;
+}
+constexpr size_t kLaneSlack = 18 * 256 + 256;  // alignment of the 18 shared buffers of a lane's layout
+
+}  // namespace
+
+struct rcm_server {
+  int device = 0;
+  rcm_server_config cfg{};
+  int32_t lanes = 4;
+  size_t lane_bytes = 0;
+  std::vector<rcm_batch*> batch;
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<ServerJob>> pending;
+  std::unordered_map<int64_t, std::shared_ptr<ServerJob>> jobs;  // not yet released by a wait
+  bool stopping = false;
+  int64_t next_id = 1;
+  int64_t inflight_bytes = 0;
+  rcm_server_stats st{};
+};
+
+namespace {
+
+// One lane: reorder a group of requests with the lane's many-matrix context.
+void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>& grp) {
+  rcm_batch* b = s->batch[(size_t)lane];
+  const int32_t C = (int32_t)grp.size();
+  std::vector<int32_t> n((size_t)C), starts((size_t)C, 0), st((size_t)C, RCM_OK), st2((size_t)C, RCM_OK);
+  std::vector<int64_t> nnz((size_t)C), rp_off((size_t)C), ci_off((size_t)C);
+  int64_t rpn = 0, cin = 0;
+  bool search = false, want_bw = false;
+  for (int32_t k = 0; k < C; ++k) {
+    n[k] = grp[k]->n;
+    nnz[k] = grp[k]->nnz;
+    rp_off[k] = rpn;
+    ci_off[k] = cin;
+    rpn += n[k] + 1;
+    cin += nnz[k];
+    search = search || grp[k]->start_req < 0;
+    want_bw = want_bw || (grp[k]->flags & RCM_FLAG_BANDWIDTH);
+  }
+  std::vector<int32_t> rp_cat((size_t)rpn), ci_cat((size_t)std::max<int64_t>(cin, 1));
+  for (int32_t k = 0; k < C; ++k) {
+    std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat.begin() + rp_off[k]);
+    std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat.begin() + ci_off[k]);
+  }
+  int hard = rcm_batch_load(b, C, n.data(), nnz.data(), rp_off.data(), ci_off.data(), rp_cat.data(), ci_cat.data(),
+                            st.data());
+  std::vector<int32_t> iters((size_t)C, 0);
+  if (hard == RCM_EINVAL) {  // some request is malformed: report it, run the rest without it
+    std::vector<std::shared_ptr<ServerJob>> good, bad;
+    for (int32_t k = 0; k < C; ++k) (st[k] ? bad : good).push_back(grp[k]);
+    for (auto& jp : bad) {
+      jp->status = RCM_EINVAL;
+      jp->perm.clear();
+    }
+    if (!good.empty()) run_group(s, lane, good);
+    grp = good;  // the caller finishes every request of the group, malformed ones included
+    grp.insert(grp.end(), bad.begin(), bad.end());
+    return;
+  }
+  if (!hard && search) {
+    const int r = rcm_batch_find_start(b, starts.data(), iters.data(), st.data());
+    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT || r == RCM_ENOTLOADED) hard = r;
+  }
+  for (int32_t k = 0; k < C; ++k)
+    if (grp[k]->start_req >= 0) {
+      starts[k] = grp[k]->start_req;
+      st[k] = RCM_OK;
+      iters[k] = 0;
+    }
+  if (!hard) {
+    const int r = rcm_batch_run(b, starts.data(), RCM_FLAG_STABLE, st2.data());
+    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT || r == RCM_ENOTLOADED || r == RCM_EINVAL) hard = r;
+  }
+  std::vector<int64_t> bw((size_t)C, -1);
+  std::vector<int32_t> cat;
+  if (!hard && want_bw) hard = rcm_batch_bandwidth(b, bw.data());
+  if (!hard) {
+    cat.resize((size_t)std::max<int64_t>(b->total_n, 1));
+    hard = rcm_batch_read_perm(b, nullptr, cat.data());
+  }
+  int64_t off = 0;
+  for (int32_t k = 0; k < C; ++k) {
+    ServerJob& j = *grp[k];
+    j.start_used = starts[k];
+    j.sweeps = iters[k];
+    j.status = hard ? hard : (st[k] ? st[k] : st2[k]);
+    if (j.status == RCM_OK) {
+      j.perm.assign(cat.begin() + off, cat.begin() + off + j.n);
+      if (j.flags & RCM_FLAG_REVERSE) std::reverse(j.perm.begin(), j.perm.end());
+      if (j.flags & RCM_FLAG_BANDWIDTH) j.bw = bw[k];
+    }
+    off += j.n;
+  }
+}
+
+void lane_loop(rcm_server* s, int lane) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    std::vector<std::shared_ptr<ServerJob>> grp;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
+      if (s->pending.empty()) return;  // stopping, nothing left
+      // a group: first come first served, looking at most 8 requests ahead for ones that fit the lane
+      size_t used = kLaneSlack;
+      size_t skipped = 0;
+      for (size_t i = 0; i < s->pending.size() && skipped < 8 && (int32_t)grp.size() < s->cfg.slots;) {
+        const auto& jp = s->pending[i];
+        if (used + jp->bytes <= s->lane_bytes) {
+          used += jp->bytes;
+          grp.push_back(jp);
+          s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
+        } else {
+          ++i;
+          ++skipped;
+        }
+      }
+      const auto now = std::chrono::steady_clock::now();
+      for (auto& jp : grp) {
+        jp->t_admit = now;
+        jp->lane = lane;
+      }
+      s->st.admitted += (int64_t)grp.size();
+      s->st.inflight += (int32_t)grp.size();
+      s->st.max_inflight = std::max(s->st.max_inflight, s->st.inflight);
+      s->st.pending = (int32_t)s->pending.size();
+      s->inflight_bytes += (int64_t)used;
+      s->st.arena_peak_bytes = std::max(s->st.arena_peak_bytes, s->inflight_bytes);
+      if (grp.empty()) continue;  // cannot happen: every request fits an empty lane (checked at submit)
+      ++s->st.loop_iterations;
+      const size_t u = used;
+      lk.unlock();
+      run_group(s, lane, grp);
+      lk.lock();
+      s->inflight_bytes -= (int64_t)u;
+      const auto t = std::chrono::steady_clock::now();
+      for (auto& jp : grp) {
+        jp->t_done = t;
+        jp->done = true;
+        jp->rp.clear();
+        jp->rp.shrink_to_fit();
+        jp->ci.clear();
+        jp->ci.shrink_to_fit();
+        if (jp->status == RCM_OK) ++s->st.completed;
+        else ++s->st.failed;
+      }
+      s->st.inflight -= (int32_t)grp.size();
+      s->cv_done.notify_all();
+    }
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* rcm_status_string(int s) {
@@ -363,6 +613,9 @@ const char* rcm_status_string(int s) {
     case RCM_ETIMEOUT: return "device watchdog expired";
     case RCM_ECAPACITY: return "internal capacity exceeded";
     case RCM_ENOTLOADED: return "no matrix loaded";
+    case RCM_ECANCELED: return "request canceled: the server shut down";
+    case RCM_ENOTFOUND: return "unknown request id";
+    case RCM_EAGAIN: return "request not finished yet";
     default: return "unknown status";
   }
 }
@@ -680,7 +933,7 @@ int rcm_batch_destroy(rcm_batch* b) {
   if (!b) return RCM_EINVAL;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  batch_free_set(b);
+  batch_release(b);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -714,24 +967,30 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   RCM_HIP(hipSetDevice(b->device));
   const auto t0 = std::chrono::steady_clock::now();
   RCM_HIP(hipStreamSynchronize(b->stream));
-  batch_free_set(b);
+  batch_reset(b);
   const size_t N = (size_t)tot, C = (size_t)count;
-  bool ok = dev_alloc(&b->rp, (size_t)rp_end) == hipSuccess && dev_alloc(&b->ci, (size_t)ci_end) == hipSuccess &&
-            dev_alloc(&b->perm, N) == hipSuccess && dev_alloc(&b->level, N) == hipSuccess &&
-            dev_alloc(&b->aux, N) == hipSuccess && dev_alloc(&b->qinfo, N) == hipSuccess &&
-            dev_alloc(&b->claim, N) == hipSuccess && dev_alloc(&b->bstat, N + C) == hipSuccess &&
-            dev_alloc(&b->brange, N + C) == hipSuccess && dev_alloc(&b->ctl, C) == hipSuccess &&
-            dev_alloc(&b->jobs, C) == hipSuccess && dev_alloc(&b->d_order, C) == hipSuccess &&
-            dev_alloc(&b->d_starts, C) == hipSuccess && dev_alloc(&b->d_active, C) == hipSuccess &&
-            dev_alloc(&b->tail, C * 96) == hipSuccess && dev_alloc(&b->d_nnz, C) == hipSuccess &&
-            dev_alloc(&b->scratch, 3 * C) == hipSuccess && dev_alloc(&b->g, 1) == hipSuccess;
-  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&b->h_ctl), C * sizeof(Control), hipHostMallocDefault) == hipSuccess;
-  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&b->h_tail), C * 96 * sizeof(int32_t), hipHostMallocDefault) ==
-                 hipSuccess;
-  if (!ok) {
-    batch_free_set(b);
-    return RCM_ENOMEM;
-  }
+  const BatchLayout L = batch_layout((size_t)rp_end, (size_t)ci_end, N, C);
+  const int ar = batch_reserve(b, L.total, count);
+  if (ar) return ar;
+  char* base = b->arena;
+  b->rp = reinterpret_cast<int32_t*>(base + L.rp);
+  b->ci = reinterpret_cast<int32_t*>(base + L.ci);
+  b->perm = reinterpret_cast<int32_t*>(base + L.perm);
+  b->level = reinterpret_cast<int32_t*>(base + L.level);
+  b->aux = reinterpret_cast<int32_t*>(base + L.aux);
+  b->qinfo = reinterpret_cast<uint64_t*>(base + L.qinfo);
+  b->claim = reinterpret_cast<uint64_t*>(base + L.claim);
+  b->bstat = reinterpret_cast<uint64_t*>(base + L.bstat);
+  b->brange = reinterpret_cast<uint64_t*>(base + L.brange);
+  b->ctl = reinterpret_cast<Control*>(base + L.ctl);
+  b->jobs = reinterpret_cast<JobDesc*>(base + L.jobs);
+  b->d_order = reinterpret_cast<int32_t*>(base + L.order);
+  b->d_starts = reinterpret_cast<int32_t*>(base + L.starts);
+  b->d_active = reinterpret_cast<int32_t*>(base + L.active);
+  b->tail = reinterpret_cast<int32_t*>(base + L.tail);
+  b->d_nnz = reinterpret_cast<int64_t*>(base + L.nnz);
+  b->scratch = reinterpret_cast<uint32_t*>(base + L.scratch);
+  b->g = reinterpret_cast<ManyControl*>(base + L.g);
   b->count = count;
   b->n.assign(n, n + count);
   b->nnz.assign(nnz, nnz + count);
@@ -785,7 +1044,7 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   b->timing.total_n = tot;
   b->timing.total_nnz = totnnz;
   b->timing.load_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (rc) batch_free_set(b);
+  if (rc) batch_reset(b);
   return rc;
 }
 
@@ -1030,6 +1289,149 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
     for (int32_t k = 0; k < count; ++k) status_out[k] = hard && status[k] == RCM_OK ? hard : status[k];
   rcm_batch_destroy(b);
   return worst;
+}
+
+int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out) {
+  if (!out) return RCM_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return RCM_EDEVICE;
+  RCM_HIP(hipSetDevice(device));
+  rcm_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.slots < 0 || c.slots > 4096 || c.lanes < 0 || c.lanes > 16 || c.arena_bytes < 0 || c.workers < 0 ||
+      c.job_timeout_ms < 0)
+    return RCM_EINVAL;
+  if (c.slots == 0) c.slots = 256;
+  if (c.lanes == 0) c.lanes = 4;
+  if (c.job_timeout_ms == 0) c.job_timeout_ms = 60000;
+  if (c.arena_bytes == 0) {
+    size_t fr = 0, tot = 0;
+    RCM_HIP(hipMemGetInfo(&fr, &tot));
+    c.arena_bytes = (int64_t)std::min<size_t>(fr / 2, (size_t)64 << 30);
+  }
+  rcm_server* s = new (std::nothrow) rcm_server();
+  if (!s) return RCM_ENOMEM;
+  s->device = device;
+  s->cfg = c;
+  s->lanes = c.lanes;
+  s->lane_bytes = (size_t)c.arena_bytes / (size_t)c.lanes;
+  int rc = RCM_OK;
+  int wtotal = 0;
+  for (int l = 0; l < s->lanes && !rc; ++l) {
+    rcm_batch* b = nullptr;
+    rc = rcm_batch_create(device, &b);
+    if (rc) break;
+    s->batch.push_back(b);
+    const int w = c.workers > 0 ? c.workers : std::max(1, b->max_resident / s->lanes);
+    rcm_batch_set_param(b, RCM_PARAM_WORKERS, w);
+    rcm_batch_set_param(b, RCM_PARAM_TIMEOUT_MS, c.job_timeout_ms);
+    rcm_batch_set_param(b, RCM_PARAM_GRAPH, 0);  // every group is a different set
+    wtotal += std::min(w, b->max_resident);
+    rc = batch_reserve(b, s->lane_bytes, c.slots);  // the lane's arena, allocated once
+    b->arena_fixed = true;
+  }
+  if (rc) {
+    rcm_server_destroy(s);
+    return rc;
+  }
+  s->st.slots = c.slots;
+  s->st.workers = wtotal;
+  s->st.arena_bytes = (int64_t)s->lane_bytes * s->lanes;
+  s->st.kernel_running = 0;
+  for (int l = 0; l < s->lanes; ++l) s->threads.emplace_back(lane_loop, s, l);
+  *out = s;
+  return RCM_OK;
+}
+
+int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                      int32_t start_node, uint32_t flags, int64_t* id_out) {
+  if (!s || !id_out || n < 1 || n >= (1 << 30) || nnz < 0 || nnz > INT32_MAX || !row_ptr || (nnz > 0 && !col_idx) ||
+      start_node >= n)
+    return RCM_EINVAL;
+  const size_t bytes = job_bytes(n, nnz);
+  if (bytes + kLaneSlack > s->lane_bytes) return RCM_ENOMEM;  // can never fit a lane
+  auto jp = std::make_shared<ServerJob>();
+  jp->n = n;
+  jp->nnz = nnz;
+  jp->rp.assign(row_ptr, row_ptr + (size_t)n + 1);
+  if (nnz > 0) jp->ci.assign(col_idx, col_idx + nnz);
+  jp->start_req = start_node;
+  jp->flags = flags;
+  jp->bytes = bytes;
+  jp->t_submit = std::chrono::steady_clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return RCM_ECANCELED;
+    jp->id = s->next_id++;
+    s->jobs[jp->id] = jp;
+    s->pending.push_back(jp);
+    ++s->st.submitted;
+    s->st.pending = (int32_t)s->pending.size();
+    *id_out = jp->id;
+  }
+  s->cv_work.notify_one();
+  return RCM_OK;
+}
+
+int rcm_server_wait(rcm_server* s, int64_t id, int64_t timeout_ms, int32_t* perm_out, rcm_job_stats* stats) {
+  if (!s) return RCM_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return RCM_ENOTFOUND;
+  std::shared_ptr<ServerJob> jp = it->second;
+  auto is_done = [&] { return jp->done; };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lk, is_done);
+  else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), is_done))
+    return RCM_EAGAIN;
+  s->jobs.erase(id);
+  lk.unlock();
+  if (perm_out && jp->status == RCM_OK) std::memcpy(perm_out, jp->perm.data(), (size_t)jp->n * sizeof(int32_t));
+  if (stats) {
+    auto ms = [](std::chrono::steady_clock::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+    rcm_job_stats js{};
+    js.status = jp->status;
+    js.start_node = jp->start_used;
+    js.sweeps = jp->sweeps;
+    js.lane = jp->lane;
+    const bool admitted = jp->lane >= 0;
+    js.queue_ms = admitted ? ms(jp->t_admit - jp->t_submit) : ms(jp->t_done - jp->t_submit);
+    js.device_ms = admitted ? ms(jp->t_done - jp->t_admit) : 0.0;
+    js.latency_ms = ms(jp->t_done - jp->t_submit);
+    js.bandwidth = jp->bw;
+    *stats = js;
+  }
+  return jp->status;
+}
+
+int rcm_server_stats_get(rcm_server* s, rcm_server_stats* out) {
+  if (!s || !out) return RCM_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  return RCM_OK;
+}
+
+int rcm_server_destroy(rcm_server* s) {
+  if (!s) return RCM_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    for (auto& jp : s->pending) {  // not yet admitted: canceled
+      jp->status = RCM_ECANCELED;
+      jp->t_done = std::chrono::steady_clock::now();
+      jp->done = true;
+      ++s->st.failed;
+    }
+    s->pending.clear();
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->threads)
+    if (t.joinable()) t.join();  // admitted groups finish first
+  s->cv_done.notify_all();
+  for (rcm_batch* b : s->batch) rcm_batch_destroy(b);
+  delete s;
+  return RCM_OK;
 }
 
 }  // extern "C"

@@ -267,6 +267,7 @@ inline uint64_t __builtin_amdgcn_s_memrealtime() {
 #define __HIP_MEMORY_SCOPE_WAVEFRONT 2
 #define __HIP_MEMORY_SCOPE_WORKGROUP 3
 #define __HIP_MEMORY_SCOPE_AGENT 4
+#define __HIP_MEMORY_SCOPE_SYSTEM 5
 #endif
 #if !defined(__clang__)  // clang (also for the host) has the __hip_atomic_* builtins itself
 template <class T>

@@ -181,3 +181,15 @@ def test_batchset_from_matrices_round_trip():
         got = bs.matrix(k)
         assert np.array_equal(got.row_offsets, m.row_offsets) and np.array_equal(got.col_ids, m.col_ids)
     assert bs.perm_offsets.tolist() == [0, 20, 27]
+
+
+def test_server_create_without_gpu_fails_cleanly():
+    import ctypes as C
+
+    h = C.c_void_p()
+    cfg = _native.ServerConfigT(4, 2, 1 << 20, 0, 1000)
+    rc = _native.lib().rcm_server_create(0, C.byref(cfg), C.byref(h))
+    if rc == 0:
+        _native.lib().rcm_server_destroy(h)
+    else:
+        assert rc == P.RCM_EDEVICE

@@ -1,0 +1,103 @@
+"""GPU: the serving form (rcm_server): requests admitted while others run on the device, each result
+bit-identical to the oracle and to the same matrix reordered alone."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+import graphs as G
+import oracle_lib as O
+import parallelbatchrcm_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _mix(count, seed):
+    bs = P.workloads.batch_mix(count, n_lo=1000, n_hi=30000, seed=seed)
+    mats = [bs.matrix(k) for k in range(len(bs))]
+    mats.insert(3, P.connect_components(P.workloads.rmat(14, seed=seed))[0])
+    mats.insert(7, P.CSR(*G.hub_graph(5000, 9000, seed=seed)))
+    return mats
+
+
+def test_server_results_match_oracle_in_the_mix():
+    mats = _mix(20, 21)
+    _say("server mix", len(mats))
+    with P.RcmServer(slots=6, lanes=3, job_timeout_ms=20000) as srv:
+        ids = [srv.submit(m, -1, P.RCM_FLAG_BANDWIDTH) for m in mats]  # all at once: more than the slots
+        out = [srv.wait(i, 60000) for i in ids]
+        st = srv.stats()
+    assert st.completed == len(mats) and st.failed == 0
+    assert st.max_inflight > 1  # requests overlapped on the device
+    for m, (rc, perm, js) in zip(mats, out):
+        assert rc == 0
+        rc0, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+        want = O.cm(m.row_offsets, m.col_ids, s)
+        assert js.start_node == s
+        assert np.array_equal(perm, want)
+        assert js.bandwidth == O.bandwidth(m.row_offsets, m.col_ids, want)
+
+
+def test_server_same_result_alone_and_in_the_mix():
+    """Each request's order alone (one at a time, nothing else on the GPU) equals its order in a mix that
+    arrives while other requests run."""
+    mats = _mix(10, 22)
+    _say("server alone vs mix")
+    with P.RcmServer(slots=4, lanes=2) as srv:
+        alone = []
+        for m in mats:
+            rc, perm, _ = srv.wait(srv.submit(m, -1, P.RCM_FLAG_REVERSE), 60000)
+            assert rc == 0
+            alone.append(perm)
+        ids = []
+        for k, m in enumerate(mats):  # staggered arrivals
+            ids.append(srv.submit(m, -1, P.RCM_FLAG_REVERSE))
+            time.sleep(0.001 * (k % 3))
+        mixed = [srv.wait(i, 60000) for i in ids]
+    for k in range(len(mats)):
+        assert mixed[k][0] == 0 and np.array_equal(mixed[k][1], alone[k]), k
+    with P.RcmContext() as c:  # and the plain single-matrix path
+        for k in (0, 3, 7):
+            c.load(mats[k])
+            s = c.find_start()
+            c.run(s, P.RCM_FLAG_REVERSE)
+            assert np.array_equal(c.perm(), alone[k])
+
+
+def test_server_errors_are_per_request():
+    rp, ci = G.from_edges(10, [(0, 1), (1, 2), (5, 6)])
+    good = P.CSR(*G.grid2d(9, 7))
+    bad_rp = np.array([0, 2, 1], np.int32)  # decreasing row offsets
+    _say("server per-request errors")
+    with P.RcmServer(slots=2, lanes=2) as srv:
+        a = srv.submit(P.CSR(rp, ci), 0)
+        b = srv.submit(P.CSR(bad_rp, np.array([0, 1], np.int32)), 0)
+        c = srv.submit(good, 0)
+        assert srv.wait(a, 30000)[0] == P.RCM_ENOTCONNECTED
+        assert srv.wait(b, 30000)[0] == P.RCM_EINVAL
+        rc, perm, _ = srv.wait(c, 30000)
+        assert rc == 0 and np.array_equal(perm, O.cm(good.row_offsets, good.col_ids, 0))
+
+
+def test_server_admission_by_arena_memory():
+    """With one lane whose arena holds only a couple of requests at once, the others wait for memory and
+    still finish: the slot limit (16) would have taken all 10 in one group."""
+    mats = _mix(8, 23)
+    need = max(4 * m.nnz + 44 * m.rows + 4096 for m in mats)
+    arena = 2 * need + 8192
+    _say("server small arena")
+    with P.RcmServer(slots=16, lanes=1, arena_bytes=arena) as srv:
+        ids = [srv.submit(m, 0) for m in mats]
+        res = [srv.wait(i, 60000) for i in ids]
+        st = srv.stats()
+    assert st.completed == len(mats)
+    assert st.loop_iterations >= 2
+    assert st.arena_peak_bytes <= arena
+    for m, (rc, perm, _) in zip(mats, res):
+        assert rc == 0 and np.array_equal(perm, O.cm(m.row_offsets, m.col_ids, 0))
