@@ -4,15 +4,15 @@
 // single-matrix run (engine.hpp); the batches of one job are processed by process_batch unchanged. What
 // differs is how a workgroup finds its next batch:
 //
-//   * Non-blocking tickets. A workgroup takes batch t of job j only when that batch already exists: it reads
-//     the job's ticket t and the record brange[t], and claims the batch with a compare-and-swap t -> t+1 on the
-//     ticket. Batch ids are still handed out in queue order, so every predecessor of a claimed batch is held
-//     by a running workgroup; no workgroup ever waits for a batch nobody can produce, whatever the residency.
-//   * Wave-wide probing. Wave 0 probes 64 candidates per step: lane 0 the workgroup's current job, lanes
-//     1..63 consecutive jobs of the begun set from a per-workgroup cursor. The lowest ready lane tries its
-//     compare-and-swap; a lost race tries the next ready lane.
-//   * Job admission. When nothing probed is ready, the workgroup begins the next job of the launch's order
-//     (largest first, so long chains start early); once every job has begun it keeps probing.
+//   * Helpers. A workgroup works for one job at a time (its "helper"); a job accepts at most cap helpers
+//     (about one per 8192 nodes, at least one). A helper takes the job's tickets with atomicAdd, exactly as
+//     the single-matrix kernel, and waits for its batch to be published; it leaves when the job is done. Every
+//     ticket below a held ticket was taken by a running workgroup, and the smallest unprocessed ticket's batch
+//     exists or never will (all earlier batches are complete), so no wait lasts forever.
+//   * Joining. A free workgroup's wave 0 probes 63 jobs of the begun set per step (from a per-workgroup
+//     cursor) and joins the first one below its cap whose next batch is already published; otherwise it
+//     begins the next job of the launch's order (largest first, so long chains start early); otherwise it
+//     joins any job below its cap; otherwise it backs off.
 //   * Completion. The first mark of a job's `done` flag (complete or failed) increments `jobs_done`; the
 //     launch ends when it reaches the job count or the watchdog sets `abort`.
 #pragma once
@@ -33,7 +33,7 @@ struct JobDesc {
   int32_t* aux;    // n entries: inverse permutation / reversed order
   Control* ctl;
   int32_t n;
-  int32_t pad;
+  int32_t cap;  // many-matrix launches: at most this many workgroups help this job at once
 };
 
 struct ManyControl {
@@ -45,6 +45,7 @@ struct ManyControl {
 
 struct ManyArgs {
   const JobDesc* jobs;
+  uint32_t* helpers;      // per job (indexed by job id): workgroups working for it
   const int32_t* order;   // the jobs of this launch, in the order they are begun
   const int32_t* starts;  // start node per job (indexed by job id)
   int32_t count;          // entries of `order`
@@ -83,8 +84,8 @@ __device__ __forceinline__ RunArgs job_args(const ManyArgs& m, int32_t j, bool b
   return a;
 }
 
-// Wave 0 of a workgroup: find and claim the next batch. Sets sm.quit = 1 when the launch is over; otherwise
-// ms.cur = job and sm.b / sm.s / sm.e = the claimed batch.
+// Wave 0 of a workgroup: the next batch. Sets sm.quit = 1 when the launch is over; otherwise ms.cur = job and
+// sm.b / sm.s / sm.e = the batch. ms.cur persists across calls while the workgroup helps that job.
 template <class G, int DIGIT>
 __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t deadline) {
   Smem<G, DIGIT>& sm = ms.s;
@@ -92,6 +93,7 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
   int32_t cur = ms.cur;
   uint32_t cursor = ms.cursor;
   const uint32_t count = (uint32_t)m.count;
+  uint32_t idle = 0;  // consecutive rounds without work: backoff grows with it
   for (;;) {
     const uint32_t fin = wave_uniform(ld_sync(&m.g->jobs_done));
     const uint32_t ab = wave_uniform(ld_sync(&m.g->abort));
@@ -101,59 +103,90 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
       stop = true;
     }
     if (stop) {
-      if (lane == 0) sm.quit = 1;
+      if (lane == 0) {
+        if (cur >= 0) atomic_add_u32(&m.helpers[cur], 0xFFFFFFFFu);
+        ms.cur = -1;
+        sm.quit = 1;
+      }
       return;
     }
+    if (cur >= 0) {
+      // helping job cur: the next ticket, then wait for its batch (or for the job to finish)
+      const JobDesc& J = m.jobs[cur];
+      uint32_t t = 0;
+      if (lane == 0) t = atomic_add_u32(&J.ctl->ticket, 1u);
+      t = wave_uniform(t);
+      for (uint32_t spin = 0;; ++spin) {
+        uint64_t r = 0;
+        if (t <= (uint32_t)J.n) r = ld_sync(&J.brange[t]);
+        r = ((uint64_t)wave_uniform((uint32_t)(r >> 32)) << 32) | wave_uniform((uint32_t)r);
+        if (r) {
+          if (lane == 0) {
+            ms.cur = cur;
+            ms.cursor = cursor;
+            sm.b = (int32_t)t;
+            sm.s = (int32_t)(uint32_t)r;
+            sm.e = (int32_t)(r >> 32);
+            sm.quit = 0;
+          }
+          return;
+        }
+        if (wave_uniform(ld_sync(&J.ctl->done))) break;  // no batch t: the job has finished
+        if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) break;
+        if (wave_uniform(ld_sync(&m.g->abort))) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (lane == 0) atomic_add_u32(&m.helpers[cur], 0xFFFFFFFFu);  // leave the job
+      cur = -1;
+      continue;
+    }
+    // free: join a job
     const uint32_t begun = min(wave_uniform(ld_sync(&m.g->next_job)), count);
     int32_t j = -1;
-    if (lane == 0)
-      j = cur;
-    else if (begun > 0)
+    bool open = false, ready = false;
+    if (lane >= 1 && (uint32_t)lane <= begun) {  // each begun job at most once per round
       j = m.order[(cursor + (uint32_t)lane - 1u) % begun];
-    uint32_t t = 0;
-    uint64_t r = 0;
-    bool ready = false;
-    if (j >= 0) {
       const JobDesc& J = m.jobs[j];
-      if (!ld_sync(&J.ctl->done)) {
-        t = ld_sync(&J.ctl->ticket);
-        if (t <= (uint32_t)J.n) {
-          r = ld_sync(&J.brange[t]);
-          ready = r != 0;
-        }
+      if (!ld_sync(&J.ctl->done) && ld_sync(&m.helpers[j]) < (uint32_t)J.cap) {
+        open = true;
+        const uint32_t t = ld_sync(&J.ctl->ticket);
+        ready = t <= (uint32_t)J.n && ld_sync(&J.brange[t]) != 0;
       }
     }
-    uint64_t mask = ballot(ready);
-    while (mask) {
-      const int l = __ffsll((unsigned long long)mask) - 1;
-      bool won = false;
-      if (lane == l) {
-        uint32_t expected = t;
-        won = __hip_atomic_compare_exchange_strong(&m.jobs[j].ctl->ticket, &expected, t + 1u, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (won) {
-          ms.cur = j;
-          ms.cursor = cursor;
-          sm.b = (int32_t)t;
-          sm.s = (int32_t)(uint32_t)r;
-          sm.e = (int32_t)(r >> 32);
-          sm.quit = 0;
-        }
-      }
-      if (ballot(won)) return;
-      mask &= mask - 1;
-    }
-    if (begun < count) {
+    cursor += kWave - 1;
+    uint64_t cand = ballot(ready);
+    if (!cand && begun < count) {
       uint32_t idx = 0;
       if (lane == 0) idx = atomic_add_u32(&m.g->next_job, 1u);
       idx = wave_uniform(idx);
       if (idx < count) {
         cur = m.order[idx];
+        if (lane == 0) atomic_add_u32(&m.helpers[cur], 1u);
+        idle = 0;
         continue;
       }
     }
-    cursor += kWave - 1;
-    __builtin_amdgcn_s_sleep(2);
+    if (!cand) cand = ballot(open);
+    bool joined = false;
+    while (cand && !joined) {
+      const int l = __ffsll((unsigned long long)cand) - 1;
+      uint32_t ok = 0;
+      if (lane == l) {
+        const uint32_t was = atomic_add_u32(&m.helpers[j], 1u);
+        ok = was < (uint32_t)m.jobs[j].cap ? 1u : 0u;
+        if (!ok) atomic_add_u32(&m.helpers[j], 0xFFFFFFFFu);  // full meanwhile
+      }
+      joined = ballot(ok != 0) != 0;
+      if (joined) cur = (int32_t)__shfl((int)j, l, kWave);
+      cand &= cand - 1;
+    }
+    if (joined) {
+      idle = 0;
+      continue;
+    }
+    // nothing to join: back off (64 to 8192 cycles) so idle workgroups do not crowd the memory channels
+    idle = min(idle + 1u, 7u);
+    for (uint32_t r = 0; r < (1u << idle); ++r) __builtin_amdgcn_s_sleep(1);
   }
 }
 
@@ -199,6 +232,7 @@ __global__ void k_init_many(ManyArgs m, int per) {
     J.brange[i] = i == 0 ? (1ull << 32) : 0ull;
   }
   if (part == 0 && threadIdx.x == 0) {
+    m.helpers[j] = 0u;
     J.perm[0] = start;
     J.qinfo[0] = (uint64_t)(uint32_t)J.row_ptr[start] | ((uint64_t)(uint32_t)J.row_ptr[start + 1] << 32);
     J.level[0] = 0;

@@ -131,7 +131,7 @@ int wait_stream(rcm_ctx* c, hipEvent_t ev, int64_t limit_ms) {
 
 int workers_for(rcm_ctx* c) {
   int w = (int)c->workers;
-  if (w <= 0) w = std::min<int64_t>(256, (int64_t)c->n / 8192 + 1);
+  if (w <= 0) w = c->n > 16384 ? 256 : 64;  // measured on MI355X: 128-256 is best from 65k to 2M nodes
   return std::max(1, std::min(w, c->max_resident));
 }
 
@@ -248,6 +248,7 @@ struct rcm_batch {
   int32_t *d_order = nullptr, *d_starts = nullptr, *d_active = nullptr, *tail = nullptr;
   int64_t* d_nnz = nullptr;
   uint32_t* scratch = nullptr;  // 3 * count words: validation flags, max row length, bandwidth
+  uint32_t* helpers = nullptr;  // count words: workgroups helping each matrix
   ManyControl* g = nullptr;
   // pinned host mirrors
   Control* h_ctl = nullptr;
@@ -269,7 +270,7 @@ constexpr int kPerJobBlocks = 4;  // helper-kernel workgroups per matrix
 // Byte layout of a loaded set in the arena: every buffer 256-byte aligned.
 struct BatchLayout {
   size_t rp, ci, perm, level, aux, qinfo, claim, bstat, brange, ctl, jobs, order, starts, active, tail, nnz,
-      scratch, g, total;
+      scratch, helpers, g, total;
 };
 BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
   auto al = [](size_t v) { return (v + 255) / 256 * 256; };
@@ -293,6 +294,7 @@ BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
   put(L.tail, 4 * 96 * C);
   put(L.nnz, 8 * C);
   put(L.scratch, 4 * 3 * C);
+  put(L.helpers, 4 * C);
   put(L.g, sizeof(ManyControl));
   L.total = o;
   return L;
@@ -352,6 +354,7 @@ int batch_reserve(rcm_batch* b, size_t bytes, int32_t count) {
 ManyArgs batch_args(rcm_batch* b, const int32_t* d_order, int32_t count) {
   ManyArgs m{};
   m.jobs = b->jobs;
+  m.helpers = b->helpers;
   m.order = d_order;
   m.starts = b->d_starts;
   m.count = count;
@@ -990,6 +993,7 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   b->tail = reinterpret_cast<int32_t*>(base + L.tail);
   b->d_nnz = reinterpret_cast<int64_t*>(base + L.nnz);
   b->scratch = reinterpret_cast<uint32_t*>(base + L.scratch);
+  b->helpers = reinterpret_cast<uint32_t*>(base + L.helpers);
   b->g = reinterpret_cast<ManyControl*>(base + L.g);
   b->count = count;
   b->n.assign(n, n + count);
@@ -1011,7 +1015,7 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
     J.brange = b->brange + pb;
     J.ctl = b->ctl + k;
     J.n = n[k];
-    J.pad = 0;
+    J.cap = (int32_t)std::min<int64_t>((int64_t)n[k] / 8192 + 1, 1024);  // helpers: ~1 per 8192 nodes
     po += n[k];
     pb += n[k] + 1;
   }

@@ -28,6 +28,7 @@ int emu_run_many(int32_t count, const int32_t* n, const int64_t* nnz, const int6
   std::vector<int32_t> perm(tot, -1), level(tot, -1), aux(tot, 0);
   std::vector<uint64_t> claim(tot), qinfo(tot), bstat(totb), brange(totb);
   std::vector<Control> ctl(count);
+  std::vector<uint32_t> helpers(count, 0);
   std::vector<JobDesc> jobs(count);
   std::vector<int32_t> order(count), starts(start, start + count);
   int64_t po = 0, pb = 0;
@@ -44,6 +45,7 @@ int emu_run_many(int32_t count, const int32_t* n, const int64_t* nnz, const int6
     J.brange = brange.data() + pb;
     J.ctl = &ctl[k];
     J.n = n[k];
+    J.cap = n[k] / 64 + 1;  // small graphs: several helpers per job, to exercise the shared tickets
     po += n[k];
     pb += n[k] + 1;
     order[k] = count - 1 - k;  // begin from the last job: any order must give the same results
@@ -51,6 +53,7 @@ int emu_run_many(int32_t count, const int32_t* n, const int64_t* nnz, const int6
   ManyControl g{};
   ManyArgs m{};
   m.jobs = jobs.data();
+  m.helpers = helpers.data();
   m.order = order.data();
   m.starts = starts.data();
   m.count = count;
