@@ -212,16 +212,29 @@ __global__ __launch_bounds__(G::T) void k_reorder_many(ManyArgs m) {
   }
 }
 
-// ---- helper kernels over the jobs of a launch: `per` workgroups per job (blockIdx.x = job slot * per + part)
+// ---- helper kernels over the jobs of a list (`order`, `count` entries) ------------------------------------
+// Workgroups are shared out by size: job slot k owns blocks [blk[k], blk[k+1]) of the grid.
+__device__ __forceinline__ int32_t slot_of_block(const int32_t* blk, int32_t count, int32_t b, int32_t* part,
+                                                 int32_t* per) {
+  int32_t lo = 0, hi = count;  // blk[0] = 0 <= b < blk[count]
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (blk[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  *part = b - blk[lo];
+  *per = blk[lo + 1] - blk[lo];
+  return lo;
+}
 
 // Reset of every job in `order` for a run from starts[job]; block 0 also resets the launch control.
-__global__ void k_init_many(ManyArgs m, int per) {
-  const int slot = (int)blockIdx.x / per, part = (int)blockIdx.x % per;
+__global__ void k_init_many(ManyArgs m, const int32_t* blk) {
+  int32_t part, per;
+  const int32_t slot = slot_of_block(blk, m.count, (int32_t)blockIdx.x, &part, &per);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ManyControl z = {};
     *m.g = z;
   }
-  if (slot >= m.count) return;
   const int32_t j = m.order[slot];
   const JobDesc& J = m.jobs[j];
   const int32_t start = m.starts[j];
@@ -243,9 +256,10 @@ __global__ void k_init_many(ManyArgs m, int per) {
 }
 
 // CSR validation per job: out[2*job] = error flags (as k_validate), out[2*job+1] = max row length.
-__global__ void k_validate_many(const JobDesc* jobs, int32_t count, int per, const int64_t* nnz, uint32_t* out) {
-  const int j = (int)blockIdx.x / per, part = (int)blockIdx.x % per;
-  if (j >= count) return;
+__global__ void k_validate_many(const JobDesc* jobs, const int32_t* order, int32_t count, const int32_t* blk,
+                                const int64_t* nnz, uint32_t* out) {
+  int32_t part, per;
+  const int32_t j = order[slot_of_block(blk, count, (int32_t)blockIdx.x, &part, &per)];
   const JobDesc& J = jobs[j];
   const int64_t stride = (int64_t)per * blockDim.x;
   const int64_t t0 = (int64_t)part * blockDim.x + threadIdx.x;
@@ -265,20 +279,19 @@ __global__ void k_validate_many(const JobDesc* jobs, int32_t count, int per, con
   if (lane_id() == 0 && mx) atomicMax(&out[2 * j + 1], mx);
 }
 
-// aux = inverse of perm (per job in `order`).
-__global__ void k_inverse_many(const JobDesc* jobs, const int32_t* order, int32_t count, int per) {
-  const int slot = (int)blockIdx.x / per, part = (int)blockIdx.x % per;
-  if (slot >= count) return;
-  const JobDesc& J = jobs[order[slot]];
+// aux = inverse of perm.
+__global__ void k_inverse_many(const JobDesc* jobs, const int32_t* order, int32_t count, const int32_t* blk) {
+  int32_t part, per;
+  const JobDesc& J = jobs[order[slot_of_block(blk, count, (int32_t)blockIdx.x, &part, &per)]];
   for (int64_t i = (int64_t)part * blockDim.x + threadIdx.x; i < J.n; i += (int64_t)per * blockDim.x)
     J.aux[J.perm[i]] = (int32_t)i;
 }
 
 // Bandwidth per job from aux (= inverse permutation): out[job] = max |inv[i] - inv[j]| over stored entries.
-__global__ void k_bandwidth_many(const JobDesc* jobs, const int32_t* order, int32_t count, int per, uint32_t* out) {
-  const int slot = (int)blockIdx.x / per, part = (int)blockIdx.x % per;
-  if (slot >= count) return;
-  const int32_t j = order[slot];
+__global__ void k_bandwidth_many(const JobDesc* jobs, const int32_t* order, int32_t count, const int32_t* blk,
+                                 uint32_t* out) {
+  int32_t part, per;
+  const int32_t j = order[slot_of_block(blk, count, (int32_t)blockIdx.x, &part, &per)];
   const JobDesc& J = jobs[j];
   uint32_t bw = 0;
   for (int64_t i = (int64_t)part * blockDim.x + threadIdx.x; i < J.n; i += (int64_t)per * blockDim.x) {
@@ -292,11 +305,10 @@ __global__ void k_bandwidth_many(const JobDesc* jobs, const int32_t* order, int3
   if (lane_id() == 0 && bw) atomicMax(&out[j], bw);
 }
 
-// aux = perm reversed (per job in `order`).
-__global__ void k_reverse_many(const JobDesc* jobs, const int32_t* order, int32_t count, int per) {
-  const int slot = (int)blockIdx.x / per, part = (int)blockIdx.x % per;
-  if (slot >= count) return;
-  const JobDesc& J = jobs[order[slot]];
+// aux = perm reversed.
+__global__ void k_reverse_many(const JobDesc* jobs, const int32_t* order, int32_t count, const int32_t* blk) {
+  int32_t part, per;
+  const JobDesc& J = jobs[order[slot_of_block(blk, count, (int32_t)blockIdx.x, &part, &per)]];
   for (int64_t i = (int64_t)part * blockDim.x + threadIdx.x; i < J.n; i += (int64_t)per * blockDim.x)
     J.aux[i] = J.perm[J.n - 1 - i];
 }

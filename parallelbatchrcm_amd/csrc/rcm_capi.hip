@@ -249,6 +249,8 @@ struct rcm_batch {
   int64_t* d_nnz = nullptr;
   uint32_t* scratch = nullptr;  // 3 * count words: validation flags, max row length, bandwidth
   uint32_t* helpers = nullptr;  // count words: workgroups helping each matrix
+  int32_t *blk_order = nullptr, *blk_active = nullptr;  // helper-kernel block maps of `order` / a sub-list
+  int32_t nblk_order = 0;
   ManyControl* g = nullptr;
   // pinned host mirrors
   Control* h_ctl = nullptr;
@@ -265,12 +267,22 @@ struct rcm_batch {
 
 namespace {
 
-constexpr int kPerJobBlocks = 4;  // helper-kernel workgroups per matrix
+// Helper-kernel grid of a list of matrices: about one workgroup per 16k rows + entries each, 1..256, as a
+// prefix map (slot k owns blocks [blk[k], blk[k+1])). Returns the grid size.
+int32_t build_blk(const rcm_batch* b, const int32_t* list, int32_t count, std::vector<int32_t>& blk) {
+  blk.assign((size_t)count + 1, 0);
+  for (int32_t i = 0; i < count; ++i) {
+    const int32_t k = list[i];
+    const int64_t w = ((int64_t)b->n[k] + b->nnz[k]) / 16384 + 1;
+    blk[i + 1] = blk[i] + (int32_t)std::min<int64_t>(w, 256);
+  }
+  return blk[count];
+}
 
 // Byte layout of a loaded set in the arena: every buffer 256-byte aligned.
 struct BatchLayout {
   size_t rp, ci, perm, level, aux, qinfo, claim, bstat, brange, ctl, jobs, order, starts, active, tail, nnz,
-      scratch, helpers, g, total;
+      scratch, helpers, blk_order, blk_active, g, total;
 };
 BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
   auto al = [](size_t v) { return (v + 255) / 256 * 256; };
@@ -295,6 +307,8 @@ BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
   put(L.nnz, 8 * C);
   put(L.scratch, 4 * 3 * C);
   put(L.helpers, 4 * C);
+  put(L.blk_order, 4 * (C + 1));
+  put(L.blk_active, 4 * (C + 1));
   put(L.g, sizeof(ManyControl));
   L.total = o;
   return L;
@@ -371,9 +385,9 @@ int batch_workers(rcm_batch* b) {
   return std::max(1, std::min(w, b->max_resident));
 }
 
-void enqueue_many(rcm_batch* b, const int32_t* d_order, int32_t count, bool bfs) {
+void enqueue_many(rcm_batch* b, const int32_t* d_order, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs) {
   const ManyArgs m = batch_args(b, d_order, count);
-  hipLaunchKernelGGL(k_init_many, dim3((unsigned)count * kPerJobBlocks), dim3(256), 0, b->stream, m, kPerJobBlocks);
+  hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, m, d_blk);
   void* kern = bfs ? reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, true>)
                    : reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, false>);
   ManyArgs mm = m;
@@ -994,6 +1008,8 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   b->d_nnz = reinterpret_cast<int64_t*>(base + L.nnz);
   b->scratch = reinterpret_cast<uint32_t*>(base + L.scratch);
   b->helpers = reinterpret_cast<uint32_t*>(base + L.helpers);
+  b->blk_order = reinterpret_cast<int32_t*>(base + L.blk_order);
+  b->blk_active = reinterpret_cast<int32_t*>(base + L.blk_active);
   b->g = reinterpret_cast<ManyControl*>(base + L.g);
   b->count = count;
   b->n.assign(n, n + count);
@@ -1031,9 +1047,12 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   RCM_HIP(hipMemcpyAsync(b->jobs, jobs.data(), C * sizeof(JobDesc), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemcpyAsync(b->d_nnz, nnz, C * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+  std::vector<int32_t> blk;
+  b->nblk_order = build_blk(b, b->order.data(), count, blk);
+  RCM_HIP(hipMemcpyAsync(b->blk_order, blk.data(), (C + 1) * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemsetAsync(b->scratch, 0, 3 * C * sizeof(uint32_t), b->stream));
-  hipLaunchKernelGGL(k_validate_many, dim3((unsigned)(C * kPerJobBlocks)), dim3(256), 0, b->stream, b->jobs, count,
-                     kPerJobBlocks, b->d_nnz, b->scratch);
+  hipLaunchKernelGGL(k_validate_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
+                     count, b->blk_order, b->d_nnz, b->scratch);
   std::vector<uint32_t> v(2 * C);
   RCM_HIP(hipMemcpyAsync(v.data(), b->scratch, 2 * C * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
   RCM_HIP(hipStreamSynchronize(b->stream));
@@ -1078,7 +1097,11 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
     const int32_t A = (int32_t)active.size();
     RCM_HIP(hipMemcpyAsync(b->d_starts, cur.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
     RCM_HIP(hipMemcpyAsync(b->d_active, active.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-    enqueue_many(b, b->d_active, A, true);
+    std::vector<int32_t> blk;
+    const int32_t nblk = build_blk(b, active.data(), A, blk);
+    RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                           b->stream));
+    enqueue_many(b, b->d_active, b->blk_active, nblk, A, true);
     hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
     RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
     RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
@@ -1135,10 +1158,10 @@ int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* s
   RCM_HIP(hipMemcpyAsync(b->d_starts, start, (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipEventRecord(b->ev[0], b->stream));
   auto enqueue_run = [&]() {
-    enqueue_many(b, b->d_order, C, false);
+    enqueue_many(b, b->d_order, b->blk_order, b->nblk_order, C, false);
     if (rev)
-      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)C * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
-                         b->d_order, C, kPerJobBlocks);
+      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
+                         C, b->blk_order);
   };
   if (b->use_graph) {
     hipGraphExec_t& ex = b->gexec[rev ? 1 : 0];
@@ -1190,13 +1213,17 @@ int rcm_batch_bandwidth(rcm_batch* b, int64_t* bw_out) {
   if (A > 0) {
     RCM_HIP(hipMemcpyAsync(b->d_active, b->ok_jobs.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice,
                            b->stream));
-    hipLaunchKernelGGL(k_inverse_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
-                       b->d_active, A, kPerJobBlocks);
-    hipLaunchKernelGGL(k_bandwidth_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
-                       b->d_active, A, kPerJobBlocks, bw);
+    std::vector<int32_t> blk;
+    const int32_t nblk = build_blk(b, b->ok_jobs.data(), A, blk);
+    RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                           b->stream));
+    hipLaunchKernelGGL(k_inverse_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, b->jobs, b->d_active, A,
+                       b->blk_active);
+    hipLaunchKernelGGL(k_bandwidth_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, b->jobs, b->d_active, A,
+                       b->blk_active, bw);
     if (b->reversed)  // aux held the reversed order: restore it (the bandwidth is the same either way)
-      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)A * kPerJobBlocks), dim3(256), 0, b->stream, b->jobs,
-                         b->d_active, A, kPerJobBlocks);
+      hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, b->jobs, b->d_active, A,
+                         b->blk_active);
   }
   RCM_HIP(hipEventRecord(b->ev[1], b->stream));
   std::vector<uint32_t> h((size_t)C);
