@@ -201,6 +201,47 @@ def bench_serving(requests: int, rates, seed: int, lanes: int):
     return out
 
 
+COPY_BYTES = 256 << 20  # calibration copy for the HBM traffic counters
+
+
+def pmc_probe(runs: int, claim_filter: int = -1) -> int:
+    """Workload of the rocprofv3 --pmc passes: the 128^3 stencil reorder `runs` times, then `runs` elementwise
+    kernels that read and write COPY_BYTES each (a plain copy of known traffic) to calibrate the counters."""
+    import torch
+
+    import parallelbatchrcm_amd as P
+
+    m = P.workloads.config(2, seed=SEED_C2)
+    with P.RcmContext(timeout_ms=30000) as ctx:
+        if claim_filter >= 0:
+            ctx.set_param("claim_filter", claim_filter)
+        ctx.load(m)
+        s = ctx.find_start()
+        for _ in range(runs):
+            ctx.run(s)
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.int32, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(runs):
+        torch.add(a, 1, out=b)  # an elementwise kernel: reads and writes COPY_BYTES
+    torch.cuda.synchronize()
+    print(f"pmc probe done: {runs} reorders, {runs} copies of {COPY_BYTES} bytes", flush=True)
+    return 0
+
+
+def measured_traffic():
+    """HBM bytes per k_reorder launch on the 128^3 stencil from the committed rocprofv3 --pmc summary
+    (profiles/k_reorder_stencil3d_128_traffic.md, line 'traffic_bytes_per_launch: <int>'), or None."""
+    path = os.path.join(ROOT, "profiles", "k_reorder_stencil3d_128_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +249,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", type=int, default=0, help="only the workload of the rocprofv3 --pmc passes")
+    ap.add_argument("--claim-filter", type=int, default=-1, help="claim pre-read filter 0/1/2 (-1: library default)")
     ap.add_argument("--batched", type=int, default=4096,
                     help="batched form on COUNT matrices of the mix (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
@@ -216,6 +259,8 @@ def main() -> int:
     ap.add_argument("--serving-rates", type=str, default="200,1000,4000")
     ap.add_argument("--serving-lanes", type=int, default=4)
     args = ap.parse_args()
+    if args.pmc_probe:
+        return pmc_probe(args.pmc_probe, args.claim_filter)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "The batched form shards by matrix; see DESIGN.md.", file=sys.stderr)
@@ -235,6 +280,8 @@ def main() -> int:
     ctx = P.RcmContext(timeout_ms=30000)
     if args.workers:
         ctx.set_param("workers", args.workers)
+    if args.claim_filter >= 0:
+        ctx.set_param("claim_filter", args.claim_filter)
     ctx.load(m)
     start = ctx.find_start()
     ctx.run(start, P.RCM_FLAG_REVERSE)
@@ -274,7 +321,7 @@ def main() -> int:
         "config": {"workload": "3D 7-point stencil 128^3, n=2097152, nnz=14581760, stable CM, GPU start search",
                    "n": n, "nnz": nnz, "start_node": start, "workers": last.workers, "batches": last.batches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_reorder (persistent)", "kernel_ms": kern_ms, "alg_bytes": balg},
         "kernel_nnz_per_s": nnz / (kern_ms * 1e-3),
         "kernel_ms_p50": float(np.percentile(kern, 50)),

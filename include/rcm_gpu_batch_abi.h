@@ -70,7 +70,9 @@ enum rcm_param {
   RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads, <= 256 parents, <= 2048 adjacency slots */
   RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
   RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
-  RCM_PARAM_GRAPH = 8        /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
+  RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
+  RCM_PARAM_CLAIM_FILTER = 9 /* 0 = atomicMin on every adjacency entry; 1 = read the claim word first and skip
+                                entries already claimed by a smaller code (plain load); 2 = same, device-scope */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);

@@ -45,6 +45,7 @@ PARAM_VARIANT = 5
 PARAM_STRICT = 6
 PARAM_PROFILE = 7
 PARAM_GRAPH = 8
+PARAM_CLAIM_FILTER = 9
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -298,7 +299,7 @@ class RcmContext:
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT,
                "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT,
-               "profile": PARAM_PROFILE}
+               "profile": PARAM_PROFILE, "claim_filter": PARAM_CLAIM_FILTER}
 
     def set_param(self, name: str, value: int) -> None:
         _check(self._lib.rcm_ctx_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
@@ -417,7 +418,8 @@ class RcmBatch:
         self.set_param("graph", 1 if graph else 0)
         self._bs: Optional[BatchSet] = None
 
-    _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH}
+    _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH,
+               "claim_filter": PARAM_CLAIM_FILTER}
 
     def set_param(self, name: str, value: int) -> None:
         _check(_native.lib().rcm_batch_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")

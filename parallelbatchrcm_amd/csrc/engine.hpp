@@ -74,7 +74,15 @@ struct RunArgs {
   int32_t seg_threshold;  // auto mode: segment rank when every parent row has at most this many entries
   uint64_t* prof;         // PROF builds: per-phase tick sums (kProfSlots entries)
   uint32_t* jobs_done;    // many-matrix launches: count of finished matrices (nullptr for one matrix)
+  int32_t claim_filter;   // 0: atomicMin on every entry; 1: read the claim word first (plain load), 2: same
+                          // with a device-scope load; the atomic is issued only if the word is still larger
 };
+
+// Claim word pre-read for the filter: codes only decrease, so any value read, stale or not, is >= the current
+// one; a value <= our code proves the claim lost and the atomic can be skipped.
+__device__ __forceinline__ uint64_t claim_peek(const RunArgs& a, int32_t c) {
+  return a.claim_filter == 2 ? ld_sync(&a.claim[c]) : a.claim[c];
+}
 
 // Marks the run finished (complete or failed). The first marker also counts the matrix as finished for a
 // many-matrix launch, so each matrix is counted exactly once.
@@ -377,10 +385,16 @@ __device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_
       cc[k] = j < D ? a.col_idx[rb + j] : 0;
     }
     __builtin_amdgcn_s_waitcnt(0);
+    uint64_t pre[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const uint32_t j = c0 + (uint32_t)(tid + k * T);
-      if (j < D) (void)atomic_min_u64(&a.claim[cc[k]], code_hi | j);
+      pre[k] = (j < D && a.claim_filter) ? claim_peek(a, cc[k]) : ~0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t j = c0 + (uint32_t)(tid + k * T);
+      if (j < D && pre[k] > (code_hi | j)) (void)atomic_min_u64(&a.claim[cc[k]], code_hi | j);
     }
   }
   publish_wait<STRICT>();
@@ -700,15 +714,34 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   // One wait for every column load: afterwards the atomics below issue back to back (without it the
   // waitcnt pass cannot count loads across the guarded blocks and waits for everything before each atomic).
   __builtin_amdgcn_s_waitcnt(0);
+  if (a.claim_filter) {
+    // read the claim words (and row bounds) first; atomics only where the word is still larger than our code
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    oldv[k] = 0;
-    crb[k] = 0;
-    cre[k] = 0;
-    if ((uint32_t)(tid + k * T) < D) {
-      oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
-      crb[k] = (uint32_t)a.row_ptr[cc[k]];
-      cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
+    for (int k = 0; k < EPT; ++k) {
+      oldv[k] = 0;
+      crb[k] = 0;
+      cre[k] = 0;
+      if ((uint32_t)(tid + k * T) < D) {
+        oldv[k] = claim_peek(a, cc[k]);
+        crb[k] = (uint32_t)a.row_ptr[cc[k]];
+        cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+      if ((uint32_t)(tid + k * T) < D && oldv[k] > code[k]) oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      oldv[k] = 0;
+      crb[k] = 0;
+      cre[k] = 0;
+      if ((uint32_t)(tid + k * T) < D) {
+        oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
+        crb[k] = (uint32_t)a.row_ptr[cc[k]];
+        cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
+      }
     }
   }
 #pragma unroll

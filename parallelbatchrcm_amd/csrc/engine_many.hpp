@@ -53,6 +53,7 @@ struct ManyArgs {
   uint64_t budget_ticks;
   int32_t sort_mode;
   int32_t seg_threshold;
+  int32_t claim_filter;
 };
 
 template <class G, int DIGIT>
@@ -79,6 +80,7 @@ __device__ __forceinline__ RunArgs job_args(const ManyArgs& m, int32_t j, bool b
   a.budget_ticks = m.budget_ticks;
   a.sort_mode = m.sort_mode;
   a.seg_threshold = m.seg_threshold;
+  a.claim_filter = m.claim_filter;
   a.prof = nullptr;
   a.jobs_done = &m.g->jobs_done;
   return a;

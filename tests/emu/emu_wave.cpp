@@ -21,7 +21,8 @@ extern "C" {
 // info[0] = device error, [1] = placed, [2] = batches, [3..11] = error diagnostics
 int emu_run(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_t start, int32_t workers,
             int32_t slots, int32_t sort_mode, int32_t bfs, int32_t* perm_out, int32_t* level_out, int32_t* info,
-            double timeout_s) {
+            double timeout_s, int32_t claim_filter) {
+  (void)claim_filter;  // the one-wave variant has no claim filter
   if (n <= 0 || start < 0 || start >= n || workers <= 0) return -1;
   std::vector<uint64_t> claim(n), qinfo(n), bstat(n + 1, 0), brange(n + 1, 0);
   std::vector<int32_t> perm(n, -1), level(n, -1);

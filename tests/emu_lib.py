@@ -26,12 +26,13 @@ def lib(engine: str = "block"):
             return L
         L.emu_run.restype = C.c_int
         L.emu_run.argtypes = [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                              C.c_int32, I32P, I32P, I32P, C.c_double]
+                              C.c_int32, I32P, I32P, I32P, C.c_double, C.c_int32]
         _libs[engine] = L
     return _libs[engine]
 
 
-def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=20.0, engine="block"):
+def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=20.0, engine="block",
+        claim_filter=0):
     """Returns (perm, level, info) with info = [error, placed, batches, diag...]."""
     if slots is None:
         slots = 2048 if engine == "block" else 1024
@@ -43,7 +44,7 @@ def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=
     info = np.zeros(12, np.int32)
     rc = lib(engine).emu_run(n, ci.shape[0], rp.ctypes.data_as(I32P), ci.ctypes.data_as(I32P), int(start), workers,
                              slots, sort_mode, 1 if bfs else 0, perm.ctypes.data_as(I32P),
-                             lev.ctypes.data_as(I32P), info.ctypes.data_as(I32P), timeout_s)
+                             lev.ctypes.data_as(I32P), info.ctypes.data_as(I32P), timeout_s, claim_filter)
     assert rc == 0, "emulator rejected the arguments"
     return perm, lev, info
 

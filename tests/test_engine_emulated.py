@@ -152,3 +152,18 @@ def test_long_rows_many():
     assert done == 3
     for k, (rp, ci) in enumerate(mats):
         assert info[k, 0] == 0 and np.array_equal(perms[k], O.cm(rp, ci, 0)), k
+
+
+@pytest.mark.parametrize("flt", [1, 2])
+@pytest.mark.parametrize("name", ["random300", "duplicates300", "stencil24x20"])
+def test_claim_filter(flt, name):
+    rp, ci = CASES[name]
+    for start in _starts(rp, ci):
+        perm, _, info = emu_lib.run(rp, ci, start, workers=3, engine="block", claim_filter=flt)
+        assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, start)), (name, start)
+
+
+def test_claim_filter_long_rows():
+    rp, ci = LONG["twohubs2600"]()
+    perm, _, info = emu_lib.run(rp, ci, 0, workers=3, engine="block", claim_filter=1, timeout_s=60)
+    assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, 0))

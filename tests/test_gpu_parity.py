@@ -210,3 +210,19 @@ def test_long_rows_match_oracle(ctx, name):
     ctx.run(ps)
     want = O.cm(m.row_offsets, m.col_ids, ps)
     assert ctx.bandwidth() == O.bandwidth(m.row_offsets, m.col_ids, want)
+
+
+@pytest.mark.parametrize("flt", [1, 2])
+def test_claim_filter_modes_agree(ctx, flt):
+    """The claim pre-read filter skips atomics on already-claimed nodes; the order must not change."""
+    cases = [P.workloads.config(3, scale=20000, seed=9), P.CSR(*G.hub_graph(5000, 9000, seed=1))]
+    try:
+        ctx.set_param("claim_filter", flt)
+        for m in cases:
+            rc, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+            _say("claim filter", flt, m.rows)
+            ctx.load(m)
+            ctx.run(s)
+            assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
+    finally:
+        ctx.set_param("claim_filter", 0)
