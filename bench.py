@@ -75,7 +75,7 @@ print("CPUBASE " + json.dumps(out))
     return None
 
 
-def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int):
+def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, autotune: bool = True):
     """Batched form on the 4096-matrix mix: every matrix reordered by one persistent launch (device-resident,
     hipGraph replay), against one-at-a-time plain GPU calls (rcm_ctx_run per matrix, inputs resident) on a
     seeded sample of the same matrices."""
@@ -103,6 +103,12 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int):
         tm = b.timing()
         out.update(load_ms=tm.load_ms, start_search_ms=tm.start_search_ms, start_sweeps=tm.start_sweeps,
                    end_to_end_s=e2e_s, workers=tm.workers)
+        if autotune:
+            t0 = time.perf_counter()
+            r = b.autotune(starts)
+            out["autotune"] = {"variant": r.variant, "threads": r.threads, "workers": r.workers,
+                               "candidates": r.candidates, "best_ms": r.best_ms, "default_ms": r.default_ms,
+                               "tuning_s": time.perf_counter() - t0}
         for _ in range(warmup):
             b.run(starts)
         walls, devs = [], []
@@ -251,6 +257,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", type=int, default=0, help="only the workload of the rocprofv3 --pmc passes")
     ap.add_argument("--claim-filter", type=int, default=-1, help="claim pre-read filter 0/1/2 (-1: library default)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time the default configuration instead of the autotuned one")
     ap.add_argument("--batched", type=int, default=4096,
                     help="batched form on COUNT matrices of the mix (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
@@ -291,6 +299,11 @@ def main() -> int:
     tm = ctx.timing()
     assert P.checkPermutation(perm) == 0
 
+    tune = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune(start, repeats=3)
+        tune_s = time.perf_counter() - t0
     for _ in range(args.warmup):
         ctx.run(start)
     kern = []
@@ -319,10 +332,12 @@ def main() -> int:
         "dtype": "int32",
         "data": "synthetic (seeded 3D 7-point stencil 128^3, symmetric random relabelling)",
         "config": {"workload": "3D 7-point stencil 128^3, n=2097152, nnz=14581760, stable CM, GPU start search",
-                   "n": n, "nnz": nnz, "start_node": start, "workers": last.workers, "batches": last.batches},
+                   "n": n, "nnz": nnz, "start_node": start, "workers": last.workers, "batches": last.batches,
+                   "variant": tune.variant if tune is not None else 0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_reorder (persistent)", "kernel_ms": kern_ms, "alg_bytes": balg},
+                     "kernel": "k_reorder (persistent)", "kernel_ms": kern_ms, "alg_bytes": balg,
+                     "traffic_source": "profiles/k_reorder_stencil3d_128_traffic.md (variant 0, 256 workgroups)"},
         "kernel_nnz_per_s": nnz / (kern_ms * 1e-3),
         "kernel_ms_p50": float(np.percentile(kern, 50)),
         "kernel_ms_min": float(np.min(kern)),
@@ -334,11 +349,16 @@ def main() -> int:
         "bandwidth": bw,
         "generate_s": gen_s,
     }
+    if tune is not None:
+        line["autotune"] = {"variant": tune.variant, "threads": tune.threads, "workers": tune.workers,
+                            "candidates": tune.candidates, "best_ms": tune.best_ms, "default_ms": tune.default_ms,
+                            "tuning_s": tune_s}
     ctx.close()
 
     if args.batched > 0:
         print(f"[bench] batched form: {args.batched} matrices", file=sys.stderr, flush=True)
-        line["batched"] = bench_batched(args.batched, 4, args.batched_steps, 1, args.one_at_a_time_sample)
+        line["batched"] = bench_batched(args.batched, 4, args.batched_steps, 1, args.one_at_a_time_sample,
+                                        args.autotune)
 
     if args.serving > 0:
         rates = [float(x) for x in args.serving_rates.split(",") if x]

@@ -66,8 +66,13 @@ enum rcm_param {
   RCM_PARAM_WORKERS = 1,     /* persistent workgroups for one matrix (0 = autotuned) */
   RCM_PARAM_TIMEOUT_MS = 2,  /* device watchdog, default 20000 */
   RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
-  RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width: 4 */
-  RCM_PARAM_VARIANT = 5,     /* batch geometry: 0 = 256 threads, <= 256 parents, <= 2048 adjacency slots */
+  RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width of the current variant (read-only: set the variant) */
+  RCM_PARAM_VARIANT = 5,     /* engine variant (block size, batch width, radix digit):
+                                0 = 256 threads, 2048 adjacency slots, 256 parents, 4-bit digits (default)
+                                1 = 256 threads, 4096 slots, 256 parents, 4-bit digits
+                                2 = 128 threads, 1024 slots, 128 parents, 4-bit digits
+                                3 = 256 threads, 2048 slots, 256 parents, 6-bit digits
+                                4 = 512 threads, 4096 slots, 512 parents, 4-bit digits */
   RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
   RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
   RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
@@ -231,6 +236,22 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
                        const int64_t* ci_off, const int32_t* row_ptr_cat, const int32_t* col_idx_cat,
                        const int32_t* start, uint32_t flags, const int64_t* po_off, int32_t* perm_cat,
                        int64_t* bandwidth_out, int32_t* status_out);
+
+/* ---- autotuner ---- */
+/* No reference counterpart. Times every candidate (engine variant x persistent workgroups) on the loaded work
+ * (`repeats` runs each, median of the device time) and keeps the fastest in the context; the order never
+ * depends on the choice. rcm_ctx_autotune runs the matrix from `start`; rcm_batch_autotune runs the whole
+ * loaded set from start[k]. */
+typedef struct rcm_tune_result {
+  int32_t variant;      /* chosen RCM_PARAM_VARIANT */
+  int32_t workers;      /* chosen RCM_PARAM_WORKERS */
+  int32_t candidates;   /* configurations timed */
+  int32_t threads;      /* block size of the chosen variant */
+  double best_ms;       /* median device time of the choice */
+  double default_ms;    /* median device time of variant 0 with automatic workers */
+} rcm_tune_result;
+int rcm_ctx_autotune(rcm_ctx* ctx, int32_t start, int32_t repeats, rcm_tune_result* out);
+int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_tune_result* out);
 
 const char* rcm_status_string(int status);
 

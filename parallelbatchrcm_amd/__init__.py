@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native
-from ._native import I32P, I64P, BatchTimingT, CsrT, JobStatsT, ServerConfigT, ServerStatsT, TimingT
+from ._native import I32P, I64P, BatchTimingT, CsrT, JobStatsT, ServerConfigT, ServerStatsT, TimingT, TuneResultT
 
 RCM_FLAG_REVERSE = 1
 RCM_FLAG_STABLE = 2
@@ -331,6 +331,12 @@ class RcmContext:
         _check(self._lib.rcm_ctx_load(self._h, m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids)), "load")
         self.n = m.rows
 
+    def autotune(self, start: int, repeats: int = 3) -> TuneResultT:
+        """Time every engine variant x workgroup count on the loaded matrix and keep the fastest."""
+        r = TuneResultT()
+        _check(self._lib.rcm_ctx_autotune(self._h, int(start), repeats, C.byref(r)), "rcm_ctx_autotune")
+        return r
+
     def find_start(self) -> int:
         s, it = C.c_int32(), C.c_int32()
         _check(self._lib.rcm_ctx_find_start(self._h, C.byref(s), C.byref(it)), "find_start")
@@ -419,7 +425,13 @@ class RcmBatch:
         self._bs: Optional[BatchSet] = None
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH,
-               "claim_filter": PARAM_CLAIM_FILTER}
+               "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT}
+
+    def autotune(self, starts, repeats: int = 1) -> TuneResultT:
+        """Time every engine variant x workgroup count on the loaded set and keep the fastest."""
+        r = TuneResultT()
+        _check(_native.lib().rcm_batch_autotune(self._h, _p32(_i32(starts)), repeats, C.byref(r)), "rcm_batch_autotune")
+        return r
 
     def set_param(self, name: str, value: int) -> None:
         _check(_native.lib().rcm_batch_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")

@@ -95,6 +95,17 @@ class ServerStatsT(C.Structure):
     ]
 
 
+class TuneResultT(C.Structure):
+    _fields_ = [
+        ("variant", C.c_int32),
+        ("workers", C.c_int32),
+        ("candidates", C.c_int32),
+        ("threads", C.c_int32),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     # rcm_gpu_batch_abi.h
@@ -128,6 +139,8 @@ _SIGNATURES = {
     "rcm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, I32P, C.POINTER(JobStatsT)]),
     "rcm_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsT)]),
     "rcm_server_destroy": (C.c_int, [C.c_void_p]),
+    "rcm_ctx_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneResultT)]),
+    "rcm_batch_autotune": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.POINTER(TuneResultT)]),
     "rcm_gpu_batch_many": (
         C.c_int,
         [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],

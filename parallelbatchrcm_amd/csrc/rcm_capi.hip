@@ -25,14 +25,58 @@ using namespace rcm;
 
 namespace {
 
-using GeoSmall = Geo<256, 2048, 256>;
-constexpr int kDigit = 4;
 constexpr int kDefaultSegThreshold = 24;
 constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56% fewer HBM writes
 
-template <bool STRICT, bool BFS, bool PROF = false>
-void* reorder_kernel() {
-  return reinterpret_cast<void*>(&k_reorder<GeoSmall, kDigit, STRICT, BFS, PROF>);
+// Engine variants (RCM_PARAM_VARIANT): block size T, batch width (adjacency slots S, parents P), radix digit.
+// The order never depends on the variant (batch boundaries do not change it); the speed does.
+constexpr int kVariants = 5;
+template <int V> struct Var;
+template <> struct Var<0> { using G = Geo<256, 2048, 256>; static constexpr int D = 4; };
+template <> struct Var<1> { using G = Geo<256, 4096, 256>; static constexpr int D = 4; };
+template <> struct Var<2> { using G = Geo<128, 1024, 128>; static constexpr int D = 4; };
+template <> struct Var<3> { using G = Geo<256, 2048, 256>; static constexpr int D = 6; };
+template <> struct Var<4> { using G = Geo<512, 4096, 512>; static constexpr int D = 4; };
+constexpr int kVarThreads[kVariants] = {256, 256, 128, 256, 512};
+constexpr int kVarSlots[kVariants] = {2048, 4096, 1024, 2048, 4096};
+constexpr int kVarDigit[kVariants] = {4, 4, 4, 6, 4};
+using GeoSmall = Var<0>::G;  // the many-matrix helpers' default
+constexpr int kDigit = 4;
+
+template <int V>
+void* reorder_kernel_v(bool strict, bool bfs, bool prof) {
+  using G = typename Var<V>::G;
+  constexpr int D = Var<V>::D;
+  if (strict) return bfs ? reinterpret_cast<void*>(&k_reorder<G, D, true, true>)
+                         : reinterpret_cast<void*>(&k_reorder<G, D, true, false>);
+  if (bfs) return reinterpret_cast<void*>(&k_reorder<G, D, false, true>);
+  return prof ? reinterpret_cast<void*>(&k_reorder<G, D, false, false, true>)
+              : reinterpret_cast<void*>(&k_reorder<G, D, false, false>);
+}
+void* reorder_kernel(int v, bool strict, bool bfs, bool prof) {
+  switch (v) {
+    case 1: return reorder_kernel_v<1>(strict, bfs, prof);
+    case 2: return reorder_kernel_v<2>(strict, bfs, prof);
+    case 3: return reorder_kernel_v<3>(strict, bfs, prof);
+    case 4: return reorder_kernel_v<4>(strict, bfs, prof);
+    default: return reorder_kernel_v<0>(strict, bfs, prof);
+  }
+}
+template <int V>
+void* many_kernel_v(bool bfs) {
+  using G = typename Var<V>::G;
+  constexpr int D = Var<V>::D;
+  return bfs ? reinterpret_cast<void*>(&k_reorder_many<G, D, false, true>)
+             : reinterpret_cast<void*>(&k_reorder_many<G, D, false, false>);
+}
+void* many_kernel(int v, bool bfs) {
+  switch (v) {
+    case 1: return many_kernel_v<1>(bfs);
+    case 2: return many_kernel_v<2>(bfs);
+    case 3: return many_kernel_v<3>(bfs);
+    case 4: return many_kernel_v<4>(bfs);
+    default: return many_kernel_v<0>(bfs);
+  }
 }
 
 #define RCM_HIP(call)                         \
@@ -61,7 +105,6 @@ struct rcm_ctx {
   int64_t timeout_ms = 20000;
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
-  int64_t digit_bits = kDigit;
   int64_t variant = 0;
   int64_t strict = 0;
   int64_t profile = 0;
@@ -88,7 +131,7 @@ struct rcm_ctx {
   uint32_t* scratch = nullptr;  // 16 words: validation flags, bandwidth
   Control* h_ctl = nullptr;     // pinned
   int32_t dirty = 0;  // batch records [0, dirty) may hold values from earlier runs
-  int max_resident = 0;
+  int max_resident[kVariants] = {};  // co-resident workgroups of each variant
   // last result
   bool have_perm = false;
   bool reversed = false;
@@ -134,7 +177,7 @@ int wait_stream(rcm_ctx* c, hipEvent_t ev, int64_t limit_ms) {
 int workers_for(rcm_ctx* c) {
   int w = (int)c->workers;
   if (w <= 0) w = c->n > 16384 ? 256 : 64;  // measured on MI355X: 128-256 is best from 65k to 2M nodes
-  return std::max(1, std::min(w, c->max_resident));
+  return std::max(1, std::min(w, c->max_resident[c->variant]));
 }
 
 // One engine run from `start` (CM order, or plain BFS order with levels when bfs).
@@ -168,11 +211,9 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   RCM_HIP(hipEventRecord(c->ev[0], c->stream));
   hipLaunchKernelGGL(k_init, dim3(init_blocks), dim3(256), 0, c->stream, a, start, bclear);
   RCM_HIP(hipEventRecord(c->ev[1], c->stream));
-  void* kern = c->strict ? (bfs ? reorder_kernel<true, true>() : reorder_kernel<true, false>())
-                         : (bfs ? reorder_kernel<false, true>()
-                                : (prof ? reorder_kernel<false, false, true>() : reorder_kernel<false, false>()));
+  void* kern = reorder_kernel((int)c->variant, c->strict != 0, bfs, prof);
   void* args[] = {&a};
-  RCM_HIP(hipLaunchKernel(kern, dim3(workers), dim3(GeoSmall::T), args, 0, c->stream));
+  RCM_HIP(hipLaunchKernel(kern, dim3(workers), dim3(kVarThreads[c->variant]), args, 0, c->stream));
   RCM_HIP(hipEventRecord(c->ev[2], c->stream));
   RCM_HIP(hipMemcpyAsync(c->h_ctl, c->ctl, sizeof(Control), hipMemcpyDeviceToHost, c->stream));
   RCM_HIP(hipEventRecord(c->ev[3], c->stream));
@@ -232,7 +273,9 @@ struct rcm_batch {
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
-  int max_resident = 0;
+  int64_t variant = 0;
+  int max_resident = 0;              // of the current variant
+  int max_res[kVariants] = {};       // co-resident workgroups of each variant
   // the loaded set
   int32_t count = 0;
   std::vector<int32_t> n;
@@ -318,14 +361,19 @@ BatchLayout batch_layout(size_t rp_end, size_t ci_end, size_t N, size_t C) {
   return L;
 }
 
-// Forget the loaded set (its captured graphs refer to its buffers); the memory is kept for the next load.
-void batch_reset(rcm_batch* b) {
+// Captured run graphs bake in the kernel, grid and arguments: drop them when any of those changes.
+void batch_drop_graphs(rcm_batch* b) {
   for (int r = 0; r < 2; ++r) {
     if (b->gexec[r]) (void)hipGraphExecDestroy(b->gexec[r]);
     if (b->graph[r]) (void)hipGraphDestroy(b->graph[r]);
     b->gexec[r] = nullptr;
     b->graph[r] = nullptr;
   }
+}
+
+// Forget the loaded set (its captured graphs refer to its buffers); the memory is kept for the next load.
+void batch_reset(rcm_batch* b) {
+  batch_drop_graphs(b);
   b->count = 0;
   b->have_perm = false;
 }
@@ -393,11 +441,10 @@ int batch_workers(rcm_batch* b) {
 void enqueue_many(rcm_batch* b, const int32_t* d_order, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs) {
   const ManyArgs m = batch_args(b, d_order, count);
   hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, m, d_blk);
-  void* kern = bfs ? reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, true>)
-                   : reinterpret_cast<void*>(&k_reorder_many<GeoSmall, kDigit, false, false>);
+  void* kern = many_kernel((int)b->variant, bfs);
   ManyArgs mm = m;
   void* args[] = {&mm};
-  (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(GeoSmall::T), args, 0, b->stream);
+  (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
 }
 
 // Per-matrix status from the control blocks of the last launch (jobs listed in `order`).
@@ -656,16 +703,19 @@ int rcm_ctx_create(int device, rcm_ctx** out) {
   ok = ok && dev_alloc(&c->ctl, 1) == hipSuccess && dev_alloc(&c->scratch, 16) == hipSuccess &&
        dev_alloc(&c->prof, kProfSlots) == hipSuccess;
   ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(Control), hipHostMallocDefault) == hipSuccess;
-  int per_cu = 0;
   hipDeviceProp_t prop;
   ok = ok && hipGetDeviceProperties(&prop, device) == hipSuccess;
-  ok = ok && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(reorder_kernel<false, false>()),
-                                                           GeoSmall::T, 0) == hipSuccess;
+  for (int v = 0; ok && v < kVariants; ++v) {
+    // persistent workers must all be resident at once: the grid is capped by the kernel's occupancy
+    int per_cu = 0;
+    ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reorder_kernel(v, false, false, false), kVarThreads[v],
+                                                      0) == hipSuccess;
+    c->max_resident[v] = std::max(1, per_cu) * prop.multiProcessorCount;
+  }
   if (!ok) {
     rcm_ctx_destroy(c);
     return RCM_EDEVICE;
   }
-  c->max_resident = std::max(1, per_cu) * prop.multiProcessorCount;
   *out = c;
   return RCM_OK;
 }
@@ -704,8 +754,8 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; c->timeout_ms = v; return RCM_OK;
     case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; c->sort_mode = v; return RCM_OK;
     case RCM_PARAM_CLAIM_FILTER: if (v < 0 || v > 2) return RCM_EINVAL; c->claim_filter = v; return RCM_OK;
-    case RCM_PARAM_DIGIT_BITS: if (v != kDigit) return RCM_EINVAL; c->digit_bits = v; return RCM_OK;
-    case RCM_PARAM_VARIANT: if (v != 0) return RCM_EINVAL; c->variant = v; return RCM_OK;
+    case RCM_PARAM_DIGIT_BITS: return v == kVarDigit[c->variant] ? RCM_OK : RCM_EINVAL;  // part of the variant
+    case RCM_PARAM_VARIANT: if (v < 0 || v >= kVariants) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
     case RCM_PARAM_PROFILE: c->profile = v ? 1 : 0; return RCM_OK;
     default: return RCM_EINVAL;
@@ -727,7 +777,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_TIMEOUT_MS: *v = c->timeout_ms; return RCM_OK;
     case RCM_PARAM_SORT: *v = c->sort_mode; return RCM_OK;
     case RCM_PARAM_CLAIM_FILTER: *v = c->claim_filter; return RCM_OK;
-    case RCM_PARAM_DIGIT_BITS: *v = c->digit_bits; return RCM_OK;
+    case RCM_PARAM_DIGIT_BITS: *v = kVarDigit[c->variant]; return RCM_OK;
     case RCM_PARAM_VARIANT: *v = c->variant; return RCM_OK;
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
     case RCM_PARAM_PROFILE: *v = c->profile; return RCM_OK;
@@ -939,16 +989,18 @@ int rcm_batch_create(int device, rcm_batch** out) {
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
   for (auto& e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
   hipDeviceProp_t prop;
-  int per_cu = 0;
   ok = ok && hipGetDeviceProperties(&prop, device) == hipSuccess;
-  ok = ok && hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                 &per_cu, reinterpret_cast<const void*>(&k_reorder_many<GeoSmall, kDigit, false, false>), GeoSmall::T,
-                 0) == hipSuccess;
+  for (int v = 0; ok && v < kVariants; ++v) {
+    int per_cu = 0;
+    ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, many_kernel(v, false), kVarThreads[v], 0) ==
+         hipSuccess;
+    b->max_res[v] = std::max(1, per_cu) * prop.multiProcessorCount;
+  }
   if (!ok) {
     rcm_batch_destroy(b);
     return RCM_EDEVICE;
   }
-  b->max_resident = std::max(1, per_cu) * prop.multiProcessorCount;
+  b->max_resident = b->max_res[0];
   *out = b;
   return RCM_OK;
 }
@@ -968,11 +1020,21 @@ int rcm_batch_destroy(rcm_batch* b) {
 int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
   if (!b) return RCM_EINVAL;
   switch (param) {
-    case RCM_PARAM_WORKERS: if (v < 0) return RCM_EINVAL; b->workers = v; return RCM_OK;
-    case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; b->timeout_ms = v; return RCM_OK;
-    case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; b->sort_mode = v; return RCM_OK;
-    case RCM_PARAM_CLAIM_FILTER: if (v < 0 || v > 2) return RCM_EINVAL; b->claim_filter = v; return RCM_OK;
+    case RCM_PARAM_WORKERS: if (v < 0) return RCM_EINVAL; b->workers = v; batch_drop_graphs(b); return RCM_OK;
+    case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; b->timeout_ms = v; batch_drop_graphs(b); return RCM_OK;
+    case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; b->sort_mode = v; batch_drop_graphs(b); return RCM_OK;
+    case RCM_PARAM_CLAIM_FILTER:
+      if (v < 0 || v > 2) return RCM_EINVAL;
+      b->claim_filter = v;
+      batch_drop_graphs(b);
+      return RCM_OK;
     case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_VARIANT:
+      if (v < 0 || v >= kVariants) return RCM_EINVAL;
+      b->variant = v;
+      batch_drop_graphs(b);  // captured graphs hold the old launch parameters
+      b->max_resident = b->max_res[v];
+      return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
@@ -1470,6 +1532,107 @@ int rcm_server_destroy(rcm_server* s) {
   s->cv_done.notify_all();
   for (rcm_batch* b : s->batch) rcm_batch_destroy(b);
   delete s;
+  return RCM_OK;
+}
+
+// Candidate workgroup counts of a variant: fractions of its co-resident capacity (and the automatic choice).
+static std::vector<int> tune_workers(int max_res) {
+  std::vector<int> w;
+  for (int x : {max_res / 8, max_res / 4, max_res / 2, max_res})
+    if (x >= 16 && std::find(w.begin(), w.end(), x) == w.end()) w.push_back(x);
+  if (w.empty()) w.push_back(std::max(1, max_res));
+  return w;
+}
+
+static double median_of(std::vector<double> v) {
+  std::sort(v.begin(), v.end());
+  return v.empty() ? 0.0 : v[v.size() / 2];
+}
+
+int rcm_ctx_autotune(rcm_ctx* c, int32_t start, int32_t repeats, rcm_tune_result* out) {
+  if (!c || !out || repeats < 1) return RCM_EINVAL;
+  if (!c->rp || c->n == 0) return RCM_ENOTLOADED;
+  if (start < 0 || start >= c->n) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(c->device));
+  const int64_t keep_v = c->variant, keep_w = c->workers;
+  rcm_tune_result best{};
+  best.best_ms = 1e300;
+  int count = 0;
+  for (int v = 0; v < kVariants; ++v) {
+    std::vector<int> ws = tune_workers(c->max_resident[v]);
+    if (v == 0) ws.insert(ws.begin(), 0);  // 0 = automatic: the default configuration
+    for (int w : ws) {
+      c->variant = v;
+      c->workers = w;
+      std::vector<double> t;
+      for (int r = 0; r < repeats; ++r) {
+        const int rc = run_engine(c, start, false);
+        if (rc) {
+          c->variant = keep_v;
+          c->workers = keep_w;
+          return rc;
+        }
+        t.push_back(c->timing.run_kernel_ms);
+      }
+      const double ms = median_of(t);
+      ++count;
+      if (v == 0 && w == 0) best.default_ms = ms;
+      if (ms < best.best_ms) {
+        best.best_ms = ms;
+        best.variant = v;
+        best.workers = workers_for(c);
+      }
+    }
+  }
+  c->variant = best.variant;
+  c->workers = best.workers;
+  best.candidates = count;
+  best.threads = kVarThreads[best.variant];
+  c->have_perm = false;  // the tuning runs overwrote the last order
+  *out = best;
+  return RCM_OK;
+}
+
+int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_tune_result* out) {
+  if (!b || !start || !out || repeats < 1) return RCM_EINVAL;
+  if (b->count == 0) return RCM_ENOTLOADED;
+  const int64_t keep_v = b->variant, keep_w = b->workers;
+  rcm_tune_result best{};
+  best.best_ms = 1e300;
+  int count = 0;
+  std::vector<int32_t> st((size_t)b->count);
+  for (int v = 0; v < kVariants; ++v) {
+    std::vector<int> ws = tune_workers(b->max_res[v]);
+    if (v == 0) ws.insert(ws.begin(), 0);
+    for (int w : ws) {
+      rcm_batch_set_param(b, RCM_PARAM_VARIANT, v);
+      rcm_batch_set_param(b, RCM_PARAM_WORKERS, w);
+      std::vector<double> t;
+      for (int r = 0; r < repeats; ++r) {
+        const int rc = rcm_batch_run(b, start, RCM_FLAG_STABLE, st.data());
+        if (rc == RCM_EDEVICE || rc == RCM_ETIMEOUT || rc == RCM_EINVAL || rc == RCM_ENOTLOADED) {
+          rcm_batch_set_param(b, RCM_PARAM_VARIANT, keep_v);
+          rcm_batch_set_param(b, RCM_PARAM_WORKERS, keep_w);
+          return rc;
+        }
+        t.push_back(b->timing.run_ms);
+      }
+      const double ms = median_of(t);
+      ++count;
+      if (v == 0 && w == 0) best.default_ms = ms;
+      if (ms < best.best_ms) {
+        best.best_ms = ms;
+        best.variant = v;
+        best.workers = batch_workers(b);
+      }
+    }
+  }
+  rcm_batch_set_param(b, RCM_PARAM_VARIANT, best.variant);
+  rcm_batch_set_param(b, RCM_PARAM_WORKERS, best.workers);
+  best.candidates = count;
+  best.threads = kVarThreads[best.variant];
+  b->have_perm = false;
+  *out = best;
   return RCM_OK;
 }
 

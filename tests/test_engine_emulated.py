@@ -167,3 +167,21 @@ def test_claim_filter_long_rows():
     rp, ci = LONG["twohubs2600"]()
     perm, _, info = emu_lib.run(rp, ci, 0, workers=3, engine="block", claim_filter=1, timeout_s=60)
     assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, 0))
+
+
+@pytest.mark.parametrize("slots", [4096, 1024, 2049, 4097], ids=["v1_256x4096", "v2_128x1024", "v3_digit6",
+                                                                  "v4_512x4096"])
+@pytest.mark.parametrize("name", ["random300", "duplicates300", "stencil24x20"])
+def test_engine_variants(slots, name):
+    rp, ci = CASES[name]
+    for start in _starts(rp, ci)[:2]:
+        for sort_mode in (0, 1):
+            perm, _, info = emu_lib.run(rp, ci, start, workers=2, slots=slots, sort_mode=sort_mode, engine="block")
+            assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, start)), (name, start, sort_mode)
+
+
+@pytest.mark.parametrize("slots", [4096, 1024, 2049, 4097])
+def test_engine_variants_long_rows(slots):
+    rp, ci = LONG["hub5000"]()
+    perm, _, info = emu_lib.run(rp, ci, 3, workers=2, slots=slots, engine="block", timeout_s=60)
+    assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, 3))

@@ -164,3 +164,34 @@ def test_batch_with_hub_members():
     for k in range(len(bs)):
         s, want, wbw = _oracle(bs.matrix(k))
         assert starts[k] == s and np.array_equal(perms[k], want) and bw[k] == wbw, k
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+def test_batch_variants_match_oracle(variant):
+    bs = P.workloads.batch_mix(20, n_lo=500, n_hi=20000, seed=31)
+    _say("batch variant", variant)
+    with P.RcmBatch(timeout_ms=30000) as b:
+        b.set_param("variant", variant)
+        b.load(bs)
+        starts, _, st = b.find_start()
+        assert (st == 0).all()
+        assert (b.run(starts) == 0).all()
+        perms = b.perms()
+    for k in range(len(bs)):
+        s, want, _ = _oracle(bs.matrix(k))
+        assert starts[k] == s and np.array_equal(perms[k], want), k
+
+
+def test_batch_autotune_keeps_results():
+    bs = P.workloads.batch_mix(32, n_lo=1000, n_hi=30000, seed=32)
+    _say("batch autotune")
+    with P.RcmBatch(timeout_ms=30000) as b:
+        b.load(bs)
+        starts, _, _ = b.find_start()
+        r = b.autotune(starts)
+        assert 0 <= r.variant < 5 and r.candidates >= 10 and 0 < r.best_ms <= r.default_ms
+        assert (b.run(starts) == 0).all()
+        perms = b.perms()
+    for k in range(len(bs)):
+        m = bs.matrix(k)
+        assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, int(starts[k]))), k

@@ -226,3 +226,37 @@ def test_claim_filter_modes_agree(ctx, flt):
             assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
     finally:
         ctx.set_param("claim_filter", 0)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+def test_engine_variants_match_oracle(ctx, variant):
+    """Every engine variant (block size, batch width, radix digit) gives the same order."""
+    cases = [P.CSR(*G.random_connected(500, 1500, seed=4)), P.workloads.config(2, scale=24, seed=2),
+             P.CSR(*G.hub_graph(5000, 9000, seed=1)), _rmat(14, 5)]
+    try:
+        ctx.set_param("variant", variant)
+        for m in cases:
+            rc, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+            _say("variant", variant, m.rows)
+            ctx.load(m)
+            assert ctx.find_start() == s
+            ctx.run(s)
+            assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
+    finally:
+        ctx.set_param("variant", 0)
+
+
+def test_autotune_keeps_results(ctx):
+    m = P.workloads.config(3, scale=40000, seed=3)
+    rc, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+    _say("autotune ctx", m.rows)
+    ctx.load(m)
+    try:
+        r = ctx.autotune(s, repeats=2)
+        assert 0 <= r.variant < 5 and r.workers >= 1 and r.candidates >= 10
+        assert 0 < r.best_ms <= r.default_ms
+        ctx.run(s)
+        assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
+    finally:
+        ctx.set_param("variant", 0)
+        ctx.set_param("workers", 0)
