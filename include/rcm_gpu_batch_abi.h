@@ -50,7 +50,8 @@ enum rcm_status {
   RCM_ENOTLOADED = -7,     /* context has no matrix loaded */
   RCM_ECANCELED = -8,      /* serving: the server shut down before the request finished */
   RCM_ENOTFOUND = -9,      /* serving: unknown request id */
-  RCM_EAGAIN = -10         /* serving: the request has not finished within the wait's time limit */
+  RCM_EAGAIN = -10,        /* serving: the request has not finished within the wait's time limit */
+  RCM_EIO = -11            /* a matrix file could not be opened, read or written */
 };
 
 /* One-shot call on host buffers: upload, (optional GPU start search), reorder, (optional bandwidth),

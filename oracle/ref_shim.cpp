@@ -3,7 +3,9 @@
 // oracle and by bench.py's cpu_baseline leg to time the reference CPU / CPU_BATCH paths.
 //
 // Calls: RealCPU::rcm<STABLE> (src/rcm_cpu.cpp:110-215; threads == 1 && batchsize == 1 runs the serial
-// rcm_seq, anything else the threaded CPU_BATCH) and Graph<float>::prepareRCM (include/Graph.h:74-264).
+// rcm_seq, anything else the threaded CPU_BATCH), Graph<T>::prepareRCM (include/Graph.h:74-264),
+// Graph<T>::load (include/Graph.h:266-324), loadMTX/storeMTX (src/COO.cpp) with convert, and
+// loadCSR/storeCSR (src/CSR.cpp). Exceptions the reference throws become a return value of -1.
 #include <cstdint>
 #include <cstring>
 #include <iostream>
@@ -11,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include "COO.h"
+#include "CSR.h"
 #include "Graph.h"
 #include "rcm_cpu.h"
 
@@ -86,4 +90,156 @@ int64_t ref_prepare(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci
 }
 
 void ref_free(int32_t* p) { delete[] p; }
+}
+
+// ---- file formats and value-carrying prepare (T = float for is_double == 0, double otherwise) ----------
+namespace {
+template <typename T>
+int64_t export_csr(const CSR<T>& m, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci, double** vals) {
+  int32_t* r = new int32_t[m.rows + 1];
+  int32_t* c = new int32_t[m.nnz > 0 ? m.nnz : 1];
+  double* v = (double*)operator new[](sizeof(double) * (m.nnz > 0 ? m.nnz : 1));
+  for (size_t i = 0; i <= m.rows; ++i) r[i] = (int32_t)m.row_offsets[i];
+  for (size_t k = 0; k < m.nnz; ++k) {
+    c[k] = (int32_t)m.col_ids[k];
+    v[k] = (double)m.data[k];
+  }
+  *rows = (int32_t)m.rows;
+  *cols = (int32_t)m.cols;
+  *rp = r;
+  *ci = c;
+  *vals = v;
+  return (int64_t)m.nnz;
+}
+
+template <typename T>
+void import_csr(CSR<T>& m, int32_t rows, int32_t cols, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                const double* vals) {
+  m.alloc((size_t)rows, (size_t)cols, (size_t)nnz);
+  for (int64_t k = 0; k < nnz; ++k) {
+    m.col_ids[k] = (unsigned)ci[k];
+    m.data[k] = vals ? (T)vals[k] : (T)1;
+  }
+  for (int32_t i = 0; i <= rows; ++i) m.row_offsets[i] = (unsigned)rp[i];
+}
+
+template <typename T>
+int64_t mtx_load(const char* path, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci, double** vals,
+                 uint32_t* info) {
+  bool pattern, complex, symmetric, hermitian;
+  COO<T> coo = loadMTX<T>(path, pattern, complex, symmetric, hermitian);
+  CSR<T> csr;
+  convert(csr, coo);
+  *info = (pattern ? 1u : 0u) | (complex ? 2u : 0u) | (symmetric ? 4u : 0u) | (hermitian ? 8u : 0u);
+  return export_csr(csr, rows, cols, rp, ci, vals);
+}
+
+template <typename T>
+void mtx_store(const char* path, int32_t rows, int32_t cols, int64_t nnz, const int32_t* rp, const int32_t* ci,
+               const double* vals) {
+  CSR<T> csr;
+  import_csr(csr, rows, cols, nnz, rp, ci, vals);
+  COO<T> coo;
+  convert(coo, csr);
+  storeMTX(coo, path);
+}
+
+template <typename T>
+int64_t graph_load(const char* file, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci, double** vals,
+                   int32_t* ret) {
+  Graph<T> g;
+  *ret = g.load(std::string(file));
+  if (*ret < 0) return -1;
+  return export_csr(g.csr, rows, cols, rp, ci, vals);
+}
+
+template <typename T>
+int64_t prepare_values(int32_t rows, int32_t cols, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                       const double* vals, int32_t check_sym, int32_t* orows, int32_t* ocols, int32_t** orp,
+                       int32_t** oci, double** ovals) {
+  Graph<T> g;
+  import_csr(g.csr, rows, cols, nnz, rp, ci, vals);
+  g.prepareRCM(check_sym != 0);
+  return export_csr(g.csr, orows, ocols, orp, oci, ovals);
+}
+}  // namespace
+
+extern "C" {
+
+// loadMTX + convert. Returns nnz, or -1 where the reference throws.
+int64_t ref_mtx_load(const char* path, int32_t is_double, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci,
+                     double** vals, uint32_t* info, int32_t quiet) {
+  CoutSilencer s(quiet != 0);
+  try {
+    return is_double ? mtx_load<double>(path, rows, cols, rp, ci, vals, info)
+                     : mtx_load<float>(path, rows, cols, rp, ci, vals, info);
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// convert(COO, CSR) + storeMTX. Returns 0, or -1 where the reference throws.
+int32_t ref_mtx_store(const char* path, int32_t is_double, int32_t rows, int32_t cols, int64_t nnz, const int32_t* rp,
+                      const int32_t* ci, const double* vals) {
+  try {
+    if (is_double)
+      mtx_store<double>(path, rows, cols, nnz, rp, ci, vals);
+    else
+      mtx_store<float>(path, rows, cols, nnz, rp, ci, vals);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+int64_t ref_hicsr_load(const char* path, int32_t is_double, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci,
+                       double** vals) {
+  try {
+    if (is_double) return export_csr(loadCSR<double>(path), rows, cols, rp, ci, vals);
+    return export_csr(loadCSR<float>(path), rows, cols, rp, ci, vals);
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+int32_t ref_hicsr_store(const char* path, int32_t is_double, int32_t rows, int32_t cols, int64_t nnz,
+                        const int32_t* rp, const int32_t* ci, const double* vals) {
+  try {
+    if (is_double) {
+      CSR<double> m;
+      import_csr(m, rows, cols, nnz, rp, ci, vals);
+      storeCSR(m, path);
+    } else {
+      CSR<float> m;
+      import_csr(m, rows, cols, nnz, rp, ci, vals);
+      storeCSR(m, path);
+    }
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Graph<T>::load. *ret receives load's return value (0, 1 = not square, -1 = failed). Returns nnz or -1.
+int64_t ref_graph_load(const char* file, int32_t is_double, int32_t* rows, int32_t* cols, int32_t** rp, int32_t** ci,
+                       double** vals, int32_t* ret, int32_t quiet) {
+  CoutSilencer s(quiet != 0);
+  try {
+    return is_double ? graph_load<double>(file, rows, cols, rp, ci, vals, ret)
+                     : graph_load<float>(file, rows, cols, rp, ci, vals, ret);
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Graph<T>::prepareRCM(check_sym) on a CSR with values. Returns the output nnz.
+int64_t ref_prepare_values(int32_t is_double, int32_t rows, int32_t cols, int64_t nnz, const int32_t* rp,
+                           const int32_t* ci, const double* vals, int32_t check_sym, int32_t* orows, int32_t* ocols,
+                           int32_t** orp, int32_t** oci, double** ovals, int32_t quiet) {
+  CoutSilencer s(quiet != 0);
+  return is_double ? prepare_values<double>(rows, cols, nnz, rp, ci, vals, check_sym, orows, ocols, orp, oci, ovals)
+                   : prepare_values<float>(rows, cols, nnz, rp, ci, vals, check_sym, orows, ocols, orp, oci, ovals);
+}
+
+void ref_free_values(double* p) { operator delete[](p); }
 }

@@ -19,7 +19,18 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native
-from ._native import I32P, I64P, BatchTimingT, CsrT, JobStatsT, ServerConfigT, ServerStatsT, TimingT, TuneResultT
+from ._native import (
+    I32P,
+    I64P,
+    BatchTimingT,
+    CsrT,
+    JobStatsT,
+    MatrixT,
+    ServerConfigT,
+    ServerStatsT,
+    TimingT,
+    TuneResultT,
+)
 
 RCM_FLAG_REVERSE = 1
 RCM_FLAG_STABLE = 2
@@ -36,6 +47,15 @@ RCM_ENOTLOADED = -7
 RCM_ECANCELED = -8
 RCM_ENOTFOUND = -9
 RCM_EAGAIN = -10
+RCM_EIO = -11
+
+MTX_PATTERN = 1
+MTX_COMPLEX = 2
+MTX_SYMMETRIC = 4
+MTX_HERMITIAN = 8
+SOURCE_RCM_HICSR = 1
+SOURCE_HICSR = 2
+SOURCE_MTX = 3
 
 PARAM_WORKERS = 1
 PARAM_TIMEOUT_MS = 2
@@ -117,14 +137,39 @@ class Graph:
         """Graph<T>::prepareRCM (Graph.h:74-264): symmetrise (optional) and connect components.
 
         Returns the number of connecting edges added. Raises RcmError(RCM_EINVAL) where the reference's
-        repair loop would not terminate (a component whose lowest node has no entries).
+        repair loop would not terminate (a component whose lowest node has no entries). Values, when the CSR
+        carries them, follow their entries; added entries get the value 1, as in the reference.
         """
         m = self.csr
+        if m.data is not None:
+            self.csr, added = prepare_rcm(m, check_sym)
+            return added
         if check_sym:
             m = symmetrize(m)
         m, added = connect_components(m)
         self.csr = m
         return added
+
+    @classmethod
+    def load(cls, file: str, dtype=np.float32, write_cache: bool = True) -> "Graph":
+        """Graph<T>::load (Graph.h:266-324): `file`+ext+".rcm.hicsr", else +".hicsr", else the MatrixMarket
+        file, prepared with prepareRCM(!symmetric) and cached as the reference caches it. ext is "" for
+        float32 and "d_" for float64. Raises RcmError on failure; `graph.source` tells where it was found
+        (SOURCE_RCM_HICSR, SOURCE_HICSR, SOURCE_MTX) and `graph.links` how many links prepareRCM added."""
+        vb = _value_bytes(dtype)
+        out = MatrixT()
+        src = C.c_int32()
+        links = C.c_int64()
+        msg = C.create_string_buffer(512)
+        rc = _native.lib().rcm_graph_load(
+            _path(file), vb, 1 if write_cache else 0, C.byref(out), C.byref(src), C.byref(links), msg, len(msg)
+        )
+        if rc != RCM_OK:
+            raise RcmError(rc, msg.value.decode(errors="replace") or "Graph.load")
+        g = cls(_take_matrix(out, dtype))
+        g.source = int(src.value)
+        g.links = int(links.value)
+        return g
 
 
 # ---------------------------------------------------------------------------------------------------------
@@ -265,6 +310,123 @@ def connect_components(m: CSR):
     if r < 0:
         raise RcmError(RCM_EINVAL, "connect_components (the reference repair loop would not terminate)")
     return _take_csr(t), int(r)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# File formats (include/rcm_formats.h): .hicsr and MatrixMarket, COO -> CSR, prepareRCM with values
+# ---------------------------------------------------------------------------------------------------------
+def _value_bytes(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return 4
+    if dt == np.float64:
+        return 8
+    raise ValueError("dtype must be float32 (CSR<float>) or float64 (CSR<double>)")
+
+
+def _path(p) -> bytes:
+    import os
+
+    return os.fsencode(p)
+
+
+def _take_matrix(t: MatrixT, dtype=np.float64) -> CSR:
+    rows, cols, nnz = t.rows, t.cols, t.nnz
+    try:
+        rp = np.ctypeslib.as_array(t.row_ptr, shape=(rows + 1,)).copy()
+        if nnz:
+            ci = np.ctypeslib.as_array(t.col_idx, shape=(nnz,)).copy()
+            data = np.ctypeslib.as_array(t.values, shape=(nnz,)).astype(dtype) if t.values else None
+        else:
+            ci = np.zeros(0, np.int32)
+            data = np.zeros(0, dtype) if t.values else None
+    finally:
+        _native.lib().rcm_matrix_free(C.byref(t))
+    return CSR(rp, ci, data, cols=int(cols))
+
+
+def _as_matrix_t(m: CSR):
+    """MatrixT view of a CSR; returns (struct, keep-alive tuple)."""
+    vals = None if m.data is None else np.ascontiguousarray(m.data, dtype=np.float64)
+    t = MatrixT(
+        m.rows,
+        m.cols,
+        m.nnz,
+        _p32(m.row_offsets),
+        _p32(m.col_ids),
+        vals.ctypes.data_as(C.POINTER(C.c_double)) if vals is not None else None,
+    )
+    return t, (m.row_offsets, m.col_ids, vals)
+
+
+def loadMTX(path, dtype=np.float32):
+    """loadMTX<T> + convert (src/COO.cpp:88-199, src/CSR.cpp:170-208). Returns (csr, info) where info has
+    the bits MTX_PATTERN, MTX_COMPLEX, MTX_SYMMETRIC, MTX_HERMITIAN."""
+    out = MatrixT()
+    info = C.c_uint32()
+    msg = C.create_string_buffer(512)
+    rc = _native.lib().rcm_mtx_load(_path(path), _value_bytes(dtype), C.byref(out), C.byref(info), msg, len(msg))
+    if rc != RCM_OK:
+        raise RcmError(rc, msg.value.decode(errors="replace") or "loadMTX")
+    return _take_matrix(out, dtype), int(info.value)
+
+
+def storeMTX(m: CSR, path, dtype=np.float32) -> None:
+    """convert(COO, CSR) + storeMTX<T> (src/CSR.cpp:210-231, src/COO.cpp:213-251)."""
+    t, keep = _as_matrix_t(m)
+    msg = C.create_string_buffer(512)
+    rc = _native.lib().rcm_mtx_store(_path(path), C.byref(t), _value_bytes(dtype), msg, len(msg))
+    del keep
+    if rc != RCM_OK:
+        raise RcmError(rc, msg.value.decode(errors="replace") or "storeMTX")
+
+
+def loadCSR(path, dtype=np.float32) -> CSR:
+    """loadCSR<T> (src/CSR.cpp:119-151): the .hicsr binary CSR."""
+    out = MatrixT()
+    msg = C.create_string_buffer(512)
+    rc = _native.lib().rcm_hicsr_load(_path(path), _value_bytes(dtype), C.byref(out), msg, len(msg))
+    if rc != RCM_OK:
+        raise RcmError(rc, msg.value.decode(errors="replace") or "loadCSR")
+    return _take_matrix(out, dtype)
+
+
+def storeCSR(m: CSR, path, dtype=np.float32) -> None:
+    """storeCSR<T> (src/CSR.cpp:153-168)."""
+    t, keep = _as_matrix_t(m)
+    msg = C.create_string_buffer(512)
+    rc = _native.lib().rcm_hicsr_store(_path(path), C.byref(t), _value_bytes(dtype), msg, len(msg))
+    del keep
+    if rc != RCM_OK:
+        raise RcmError(rc, msg.value.decode(errors="replace") or "storeCSR")
+
+
+def convert_coo(rows: int, cols: int, row_ids, col_ids, values=None) -> CSR:
+    """convert(CSR, COO) (src/CSR.cpp:170-208): stable sort by (row, column), duplicates kept."""
+    ri = _i32(row_ids)
+    ci = _i32(col_ids)
+    if ri.shape != ci.shape:
+        raise ValueError("row_ids and col_ids differ in length")
+    vals = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+    out = MatrixT()
+    rc = _native.lib().rcm_coo_to_csr(
+        rows, cols, ri.size, _p32(ri), _p32(ci),
+        vals.ctypes.data_as(C.POINTER(C.c_double)) if vals is not None else None, C.byref(out),
+    )
+    _check(rc, "convert")
+    return _take_matrix(out)
+
+
+def prepare_rcm(m: CSR, check_sym: bool = True):
+    """prepareRCM on a CSR with values (Graph.h:74-264). Returns (csr, links_added)."""
+    t, keep = _as_matrix_t(m)
+    out = MatrixT()
+    r = _native.lib().rcm_prepare(C.byref(t), 1 if check_sym else 0, C.byref(out))
+    del keep
+    if r < 0:
+        raise RcmError(int(r), "prepareRCM")
+    dtype = m.data.dtype if m.data is not None else np.float64
+    return _take_matrix(out, dtype), int(r)
 
 
 # ---------------------------------------------------------------------------------------------------------

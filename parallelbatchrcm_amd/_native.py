@@ -1,4 +1,5 @@
-"""ctypes bindings of the engine library (include/rcm_gpu_batch_abi.h, include/rcm_workloads.h).
+"""ctypes bindings of the engine library (include/rcm_gpu_batch_abi.h, include/rcm_workloads.h,
+include/rcm_formats.h).
 
 The library is built in-tree by parallelbatchrcm_amd/build.py. Loading fails loudly when it is missing:
 there is no CPU fallback behind this package.
@@ -18,6 +19,17 @@ U64P = C.POINTER(C.c_uint64)
 
 class CsrT(C.Structure):
     _fields_ = [("n", C.c_int32), ("nnz", C.c_int64), ("row_ptr", I32P), ("col_idx", I32P)]
+
+
+class MatrixT(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int32),
+        ("cols", C.c_int32),
+        ("nnz", C.c_int64),
+        ("row_ptr", I32P),
+        ("col_idx", I32P),
+        ("values", C.POINTER(C.c_double)),
+    ]
 
 
 class TimingT(C.Structure):
@@ -167,6 +179,21 @@ _SIGNATURES = {
         ],
     ),
     "rcm_free": (None, [C.c_void_p]),
+    # rcm_formats.h
+    "rcm_matrix_free": (None, [C.POINTER(MatrixT)]),
+    "rcm_mtx_load": (C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(MatrixT), C.POINTER(C.c_uint32), C.c_char_p, C.c_size_t]),
+    "rcm_mtx_store": (C.c_int32, [C.c_char_p, C.POINTER(MatrixT), C.c_int32, C.c_char_p, C.c_size_t]),
+    "rcm_hicsr_load": (C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(MatrixT), C.c_char_p, C.c_size_t]),
+    "rcm_hicsr_store": (C.c_int32, [C.c_char_p, C.POINTER(MatrixT), C.c_int32, C.c_char_p, C.c_size_t]),
+    "rcm_coo_to_csr": (
+        C.c_int32,
+        [C.c_int32, C.c_int32, C.c_int64, I32P, I32P, C.POINTER(C.c_double), C.POINTER(MatrixT)],
+    ),
+    "rcm_prepare": (C.c_int64, [C.POINTER(MatrixT), C.c_int32, C.POINTER(MatrixT)]),
+    "rcm_graph_load": (
+        C.c_int32,
+        [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(MatrixT), I32P, I64P, C.c_char_p, C.c_size_t],
+    ),
     "rcm_splitmix64": (C.c_uint64, [U64P]),
 }
 

@@ -685,6 +685,7 @@ const char* rcm_status_string(int s) {
     case RCM_ECANCELED: return "request canceled: the server shut down";
     case RCM_ENOTFOUND: return "unknown request id";
     case RCM_EAGAIN: return "request not finished yet";
+    case RCM_EIO: return "matrix file could not be opened, read or written";
     default: return "unknown status";
   }
 }

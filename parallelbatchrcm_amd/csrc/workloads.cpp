@@ -2,6 +2,7 @@
 // include/rcm_workloads.h). Host C++17; parallel where the work is large, deterministic regardless of the
 // thread count (every random stream is tied to a fixed chunk or matrix index, never to a thread).
 #include "rcm_workloads.h"
+#include "host_canon.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -111,6 +112,62 @@ int32_t csr_from_rows(int32_t n, const std::vector<int64_t>& deg, bool dedupe, i
 }
 
 }  // namespace
+
+namespace pbrcm_host {
+
+// The edges Graph<T>::prepareRCM adds to connect the matrix (include/Graph.h:160-235): a BFS from the first
+// non-empty row; then, while nodes remain unreached, the lowest unreached node `low` is linked to low - 1
+// (or, for low == 0, to the lowest reached node) and the BFS continues from `low`. Links are returned as
+// (first, second) in the order the reference records them. Returns 0, or -3 where the reference's loop
+// would not terminate (the lowest node of a component has an empty row).
+int32_t connect_links(int32_t n, const int32_t* rp, const int32_t* ci,
+                      std::vector<std::pair<int32_t, int32_t>>& links) {
+  links.clear();
+  // first non-empty row seeds the search (Graph.h:171-176)
+  int32_t seed_row = -1;
+  for (int32_t r = 0; r < n && seed_row < 0; ++r)
+    if (rp[r + 1] > rp[r]) seed_row = r;
+  if (seed_row < 0) return -3;
+  std::vector<uint8_t> seen((size_t)n, 0);
+  std::vector<int32_t> fifo;
+  fifo.reserve((size_t)n);
+  size_t head = 0;
+  int64_t reached = 1;
+  seen[seed_row] = 1;
+  fifo.push_back(seed_row);
+  auto drain = [&]() {
+    while (head < fifo.size()) {
+      const int32_t v = fifo[head++];
+      for (int32_t t = rp[v]; t < rp[v + 1]; ++t) {
+        const int32_t c = ci[t];
+        if (!seen[c]) {
+          seen[c] = 1;
+          fifo.push_back(c);
+          ++reached;
+        }
+      }
+    }
+  };
+  int32_t low = 0;
+  drain();
+  while (reached != n) {
+    while (seen[low]) ++low;  // lowest unreached node; everything below it is reached
+    if (!links.empty() && links.back().second == low && low > 0) return -3;  // the link did not reach it
+    if (low > 0)
+      links.emplace_back(low - 1, low);
+    else {
+      int32_t j = 1;
+      while (j < n && !seen[j]) ++j;
+      if (!links.empty() && links.back().first == 0 && links.back().second == j) return -3;
+      links.emplace_back(0, j);
+    }
+    fifo.push_back(low);  // queued without being marked (Graph.h:214)
+    drain();
+  }
+  return 0;
+}
+
+}  // namespace pbrcm_host
 
 extern "C" {
 
@@ -311,48 +368,9 @@ int64_t rcm_connect_components(const rcm_csr_t* in, rcm_csr_t* out) {
   const int32_t n = in->n;
   const int32_t* rp = in->row_ptr;
   const int32_t* ci = in->col_idx;
-  // first non-empty row seeds the search (Graph.h:171-176)
-  int32_t seed_row = -1;
-  for (int32_t r = 0; r < n && seed_row < 0; ++r)
-    if (rp[r + 1] > rp[r]) seed_row = r;
-  if (seed_row < 0) return -3;
-  std::vector<uint8_t> seen((size_t)n, 0);
-  std::vector<int32_t> fifo;
-  fifo.reserve((size_t)n);
-  size_t head = 0;
-  int64_t reached = 1;
-  seen[seed_row] = 1;
-  fifo.push_back(seed_row);
-  auto drain = [&]() {
-    while (head < fifo.size()) {
-      const int32_t v = fifo[head++];
-      for (int32_t t = rp[v]; t < rp[v + 1]; ++t) {
-        const int32_t c = ci[t];
-        if (!seen[c]) {
-          seen[c] = 1;
-          fifo.push_back(c);
-          ++reached;
-        }
-      }
-    }
-  };
-  std::vector<std::pair<int32_t, int32_t>> links;  // (first, second) as the reference records them
-  int32_t low = 0;
-  drain();
-  while (reached != n) {
-    while (seen[low]) ++low;  // lowest unreached node; everything below it is reached
-    if (!links.empty() && links.back().second == low && low > 0) return -3;  // the link did not reach it
-    if (low > 0)
-      links.emplace_back(low - 1, low);
-    else {
-      int32_t j = 1;
-      while (j < n && !seen[j]) ++j;
-      if (!links.empty() && links.back().first == 0 && links.back().second == j) return -3;
-      links.emplace_back(0, j);
-    }
-    fifo.push_back(low);  // queued without being marked (Graph.h:214)
-    drain();
-  }
+  std::vector<std::pair<int32_t, int32_t>> links;
+  const int32_t lrc = pbrcm_host::connect_links(n, rp, ci, links);
+  if (lrc < 0) return lrc;
   if (links.empty()) {
     if (!alloc_csr(out, n, in->nnz)) return -1;
     std::memcpy(out->row_ptr, rp, ((size_t)n + 1) * sizeof(int32_t));

@@ -76,6 +76,23 @@ def ref() -> C.CDLL:
                                   I64P, C.c_int32]
         L.ref_free.restype = None
         L.ref_free.argtypes = [I32P]
+        DP = C.POINTER(C.c_double)
+        out3 = [I32P, I32P, C.POINTER(I32P), C.POINTER(I32P), C.POINTER(DP)]
+        L.ref_mtx_load.restype = C.c_int64
+        L.ref_mtx_load.argtypes = [C.c_char_p, C.c_int32, *out3, C.POINTER(C.c_uint32), C.c_int32]
+        L.ref_mtx_store.restype = C.c_int32
+        L.ref_mtx_store.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, I32P, I32P, DP]
+        L.ref_hicsr_load.restype = C.c_int64
+        L.ref_hicsr_load.argtypes = [C.c_char_p, C.c_int32, *out3]
+        L.ref_hicsr_store.restype = C.c_int32
+        L.ref_hicsr_store.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, I32P, I32P, DP]
+        L.ref_graph_load.restype = C.c_int64
+        L.ref_graph_load.argtypes = [C.c_char_p, C.c_int32, *out3, I32P, C.c_int32]
+        L.ref_prepare_values.restype = C.c_int64
+        L.ref_prepare_values.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, I32P, I32P, DP, C.c_int32,
+                                         *out3, C.c_int32]
+        L.ref_free_values.restype = None
+        L.ref_free_values.argtypes = [DP]
         _ref = L
     return _ref
 
@@ -175,3 +192,77 @@ def ref_prepare(rp, ci, check_sym):
     ref().ref_free(orp)
     ref().ref_free(oci)
     return r, c
+
+
+# ---- the reference's file formats (src/COO.cpp, src/CSR.cpp, include/Graph.h) ----------------------------
+def _take_ref_csr(nnz, rows, cols, rp, ci, vals):
+    r = np.ctypeslib.as_array(rp, shape=(rows.value + 1,)).copy()
+    c = np.ctypeslib.as_array(ci, shape=(max(nnz, 1),))[:nnz].copy()
+    v = np.ctypeslib.as_array(vals, shape=(max(nnz, 1),))[:nnz].copy()
+    ref().ref_free(rp)
+    ref().ref_free(ci)
+    ref().ref_free_values(vals)
+    return rows.value, cols.value, r, c, v
+
+
+def _out3():
+    return C.c_int32(), C.c_int32(), I32P(), I32P(), C.POINTER(C.c_double)()
+
+
+def ref_mtx_load(path, is_double=False):
+    """loadMTX + convert. Returns None where the reference throws, else (rows, cols, rp, ci, vals, info)."""
+    rows, cols, rp, ci, vals = _out3()
+    info = C.c_uint32()
+    nnz = ref().ref_mtx_load(os.fsencode(path), int(is_double), C.byref(rows), C.byref(cols), C.byref(rp),
+                             C.byref(ci), C.byref(vals), C.byref(info), 1)
+    if nnz < 0:
+        return None
+    return (*_take_ref_csr(nnz, rows, cols, rp, ci, vals), info.value)
+
+
+def _dp(v):
+    return v.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ref_mtx_store(path, rows, cols, rp, ci, vals, is_double=False):
+    rp, ci = as_csr(rp, ci)
+    vals = np.ascontiguousarray(vals, np.float64)
+    return int(ref().ref_mtx_store(os.fsencode(path), int(is_double), rows, cols, ci.shape[0], _p32(rp), _p32(ci),
+                                   _dp(vals)))
+
+
+def ref_hicsr_load(path, is_double=False):
+    rows, cols, rp, ci, vals = _out3()
+    nnz = ref().ref_hicsr_load(os.fsencode(path), int(is_double), C.byref(rows), C.byref(cols), C.byref(rp),
+                               C.byref(ci), C.byref(vals))
+    if nnz < 0:
+        return None
+    return _take_ref_csr(nnz, rows, cols, rp, ci, vals)
+
+
+def ref_hicsr_store(path, rows, cols, rp, ci, vals, is_double=False):
+    rp, ci = as_csr(rp, ci)
+    vals = np.ascontiguousarray(vals, np.float64)
+    return int(ref().ref_hicsr_store(os.fsencode(path), int(is_double), rows, cols, ci.shape[0], _p32(rp), _p32(ci),
+                                     _dp(vals)))
+
+
+def ref_graph_load(file, is_double=False):
+    """Graph<T>::load. Returns (ret, rows, cols, rp, ci, vals) or (ret, None...) when it failed."""
+    rows, cols, rp, ci, vals = _out3()
+    ret = C.c_int32(-1)
+    nnz = ref().ref_graph_load(os.fsencode(file), int(is_double), C.byref(rows), C.byref(cols), C.byref(rp),
+                               C.byref(ci), C.byref(vals), C.byref(ret), 1)
+    if nnz < 0:
+        return ret.value, None
+    return ret.value, _take_ref_csr(nnz, rows, cols, rp, ci, vals)
+
+
+def ref_prepare_values(rows, cols, rp, ci, vals, check_sym, is_double=False):
+    rp, ci = as_csr(rp, ci)
+    vals = np.ascontiguousarray(vals, np.float64)
+    orows, ocols, orp, oci, ovals = _out3()
+    nnz = ref().ref_prepare_values(int(is_double), rows, cols, ci.shape[0], _p32(rp), _p32(ci), _dp(vals),
+                                   1 if check_sym else 0, C.byref(orows), C.byref(ocols), C.byref(orp), C.byref(oci),
+                                   C.byref(ovals), 1)
+    return _take_ref_csr(nnz, orows, ocols, orp, oci, ovals)
