@@ -51,7 +51,8 @@ enum rcm_status {
   RCM_ECANCELED = -8,      /* serving: the server shut down before the request finished */
   RCM_ENOTFOUND = -9,      /* serving: unknown request id */
   RCM_EAGAIN = -10,        /* serving: the request has not finished within the wait's time limit */
-  RCM_EIO = -11            /* a matrix file could not be opened, read or written */
+  RCM_EIO = -11,           /* a matrix file could not be opened, read or written */
+  RCM_ENOTSYMMETRIC = -12  /* the device connectivity repair needs a structurally symmetric matrix */
 };
 
 /* One-shot call on host buffers: upload, (optional GPU start search), reorder, (optional bandwidth),

@@ -48,6 +48,7 @@ RCM_ECANCELED = -8
 RCM_ENOTFOUND = -9
 RCM_EAGAIN = -10
 RCM_EIO = -11
+RCM_ENOTSYMMETRIC = -12
 
 MTX_PATTERN = 1
 MTX_COMPLEX = 2
@@ -427,6 +428,76 @@ def prepare_rcm(m: CSR, check_sym: bool = True):
         raise RcmError(int(r), "prepareRCM")
     dtype = m.data.dtype if m.data is not None else np.float64
     return _take_matrix(out, dtype), int(r)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# GPU canonicalisation and permutation (include/rcm_matrix_ops.h)
+# ---------------------------------------------------------------------------------------------------------
+class MatrixOps:
+    """An rcm_ops handle: COO -> CSR + prepareRCM, and B = P A P^T with values, on the GPU."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(_native.lib().rcm_ops_create(device, C.byref(self._h)), "rcm_ops_create")
+
+    def close(self) -> None:
+        if self._h:
+            _native.lib().rcm_ops_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        _check(_native.lib().rcm_ops_last_ms(self._h, C.byref(ms)), "rcm_ops_last_ms")
+        return ms.value
+
+    def permute(self, m: CSR, perm) -> CSR:
+        """B = P A P^T: row i of B is row perm[i] of A, columns renamed to their positions in perm (sorted,
+        duplicates in their order in A). The device side of CuSolverRCM::reorder (rcm_cusolver.cuh:154-185)."""
+        perm = _i32(perm)
+        if perm.shape[0] != m.rows:
+            raise ValueError("perm length differs from the row count")
+        vals = None if m.data is None else np.ascontiguousarray(m.data, np.float64)
+        rp2 = np.empty(m.rows + 1, np.int32)
+        ci2 = np.empty(max(m.nnz, 1), np.int32)
+        v2 = None if vals is None else np.empty(max(m.nnz, 1), np.float64)
+        dp = C.POINTER(C.c_double)
+        rc = _native.lib().rcm_ops_permute(
+            self._h, m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids),
+            vals.ctypes.data_as(dp) if vals is not None else None, _p32(perm), _p32(rp2), _p32(ci2),
+            v2.ctypes.data_as(dp) if v2 is not None else None,
+        )
+        _check(rc, "rcm_ops_permute")
+        data = None if v2 is None else v2[: m.nnz].astype(m.data.dtype)
+        return CSR(rp2, ci2[: m.nnz], data, cols=m.rows)
+
+    def canonicalize(self, rows: int, cols: int, row_ids, col_ids, values=None, check_sym: bool = True):
+        """convert + prepareRCM(check_sym) on the GPU. Returns (csr, links_added)."""
+        ri, ci = _i32(row_ids), _i32(col_ids)
+        if ri.shape != ci.shape:
+            raise ValueError("row_ids and col_ids differ in length")
+        vals = None if values is None else np.ascontiguousarray(values, np.float64)
+        out = MatrixT()
+        links = C.c_int64()
+        rc = _native.lib().rcm_ops_canonicalize(
+            self._h, rows, cols, ri.size, _p32(ri), _p32(ci),
+            vals.ctypes.data_as(C.POINTER(C.c_double)) if vals is not None else None, 1 if check_sym else 0,
+            C.byref(out), C.byref(links),
+        )
+        _check(rc, "rcm_ops_canonicalize")
+        return _take_matrix(out), int(links.value)
 
 
 # ---------------------------------------------------------------------------------------------------------

@@ -1,5 +1,5 @@
 """ctypes bindings of the engine library (include/rcm_gpu_batch_abi.h, include/rcm_workloads.h,
-include/rcm_formats.h).
+include/rcm_formats.h, include/rcm_matrix_ops.h).
 
 The library is built in-tree by parallelbatchrcm_amd/build.py. Loading fails loudly when it is missing:
 there is no CPU fallback behind this package.
@@ -190,6 +190,34 @@ _SIGNATURES = {
         [C.c_int32, C.c_int32, C.c_int64, I32P, I32P, C.POINTER(C.c_double), C.POINTER(MatrixT)],
     ),
     "rcm_prepare": (C.c_int64, [C.POINTER(MatrixT), C.c_int32, C.POINTER(MatrixT)]),
+    # rcm_matrix_ops.h
+    "rcm_ops_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rcm_ops_destroy": (C.c_int, [C.c_void_p]),
+    "rcm_ops_last_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "rcm_ops_permute_device": (
+        C.c_int,
+        [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p],
+    ),
+    "rcm_ops_permute": (
+        C.c_int,
+        [C.c_void_p, C.c_int32, C.c_int64, I32P, I32P, C.POINTER(C.c_double), I32P, I32P, I32P,
+         C.POINTER(C.c_double)],
+    ),
+    "rcm_ops_canonicalize_device": (
+        C.c_int,
+        [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+         I64P],
+    ),
+    "rcm_ops_result": (
+        C.c_int,
+        [C.c_void_p, I32P, I64P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    ),
+    "rcm_ops_canonicalize": (
+        C.c_int,
+        [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, I32P, I32P, C.POINTER(C.c_double), C.c_int32,
+         C.POINTER(MatrixT), I64P],
+    ),
     "rcm_graph_load": (
         C.c_int32,
         [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(MatrixT), I32P, I64P, C.c_char_p, C.c_size_t],

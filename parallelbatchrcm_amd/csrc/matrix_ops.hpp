@@ -1,0 +1,790 @@
+// matrix_ops.hpp -- device kernels for the steps either side of the reorder: input canonicalisation (COO -> CSR,
+// symmetrisation, connectivity repair) and applying a symmetric permutation B = P A P^T with values.
+//
+// The host orchestration is templated on an executor `X` (launch, scratch allocation, readback), so the same
+// kernels and the same sequence run on the GPU (matrix_ops.hip) and in the CPU wave emulator of the tests.
+//
+// Building blocks:
+//   * a device-wide exclusive scan of u32 (reduce / top / apply, 4096 items per workgroup);
+//   * a stable LSD radix sort of (u64 key, u32 payload) pairs with 8-bit digits: per pass a digit histogram
+//     per 4096-item tile, a scan of the [digit][tile] table, and a scatter that ranks the tile stably in LDS
+//     (wave64 ballots on the digit, per-wave digit counts) and then writes each digit's run contiguously.
+//
+// Every kernel reads only inside the sizes it is given; malformed inputs are rejected by validation kernels
+// before any kernel that indexes by their values.
+#pragma once
+#include "device_common.hpp"
+
+namespace rcm_mx {
+
+using rcm::ballot;
+using rcm::kWave;
+using rcm::lane_id;
+using rcm::lanemask_lt;
+using rcm::wave_id;
+
+constexpr int kT = 256;              // threads per workgroup
+constexpr int kWaves = kT / kWave;   // 4
+constexpr int kItems = 16;           // items per thread in scan and sort tiles
+constexpr int kTile = kT * kItems;   // 4096
+constexpr int kBins = 256;           // 8-bit radix digits
+constexpr int kShortRow = 64;        // rows up to this length are ordered by rank counting in LDS
+constexpr int kRowsPerGroup = 64;    // rows per workgroup in that kernel (one per lane of wave 0)
+
+static_assert(kT == kBins, "one thread per digit");
+
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t* p) { return rcm::ld_sync(p); }
+__device__ __forceinline__ void st_u32(uint32_t* p, uint32_t v) { rcm::st_sync(p, v); }
+
+// ------------------------------------------------------------------------------------------------------
+// Device-wide exclusive scan of u32 (in place allowed).
+// ------------------------------------------------------------------------------------------------------
+struct ScanSmem {
+  uint32_t w[kWaves + 1];
+  uint32_t carry;
+};
+
+// partial[b] = sum of in[b * kTile, (b + 1) * kTile)
+__global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* in, int64_t m, uint32_t* partial) {
+  RCM_LDS(ScanSmem, sm);
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  uint32_t s = 0;
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = base + (int64_t)j * kT + threadIdx.x;
+    if (i < m) s += in[i];
+  }
+  uint32_t total;
+  rcm::block_excl_scan<kT>(s, sm.w, &total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of partial[0, p) in place; *total_out (nullable) receives the sum.
+__global__ __launch_bounds__(kT) void k_scan_top(uint32_t* partial, int64_t p, uint32_t* total_out) {
+  RCM_LDS(ScanSmem, sm);
+  uint32_t carry = 0;
+  for (int64_t c = 0; c < p; c += kT) {
+    const int64_t i = c + threadIdx.x;
+    const uint32_t v = i < p ? partial[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = rcm::block_excl_scan<kT>(v, sm.w, &tot);
+    if (i < p) partial[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+// out[i] = partial[tile] + exclusive prefix of in within the tile; rounds of kT consecutive items.
+__global__ __launch_bounds__(kT) void k_scan_apply(const uint32_t* in, uint32_t* out, int64_t m,
+                                                  const uint32_t* partial) {
+  RCM_LDS(ScanSmem, sm);
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  uint32_t carry = partial[blockIdx.x];
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = base + (int64_t)j * kT + threadIdx.x;
+    const uint32_t v = i < m ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = rcm::block_excl_scan<kT>(v, sm.w, &tot);
+    if (i < m) out[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Stable LSD radix sort of (u64 key, u32 payload), one 8-bit digit per pass.
+// ------------------------------------------------------------------------------------------------------
+struct HistSmem {
+  uint32_t h[kBins];
+};
+
+// hist[d * nblocks + b] = number of keys of tile b whose digit at `shift` is d
+__global__ __launch_bounds__(kT) void k_radix_hist(const uint64_t* keys, int64_t n, int shift, uint32_t* hist,
+                                                  uint32_t nblocks) {
+  RCM_LDS(HistSmem, sm);
+  const int t = threadIdx.x;
+  sm.h[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = base + (int64_t)j * kT + t;
+    if (i < n) atomicAdd(&sm.h[(uint32_t)(keys[i] >> shift) & (kBins - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)t * nblocks + blockIdx.x] = sm.h[t];
+}
+
+struct ScatterSmem {
+  uint64_t key[kTile];
+  uint32_t val[kTile];
+  uint32_t tile_off[kBins];  // exclusive prefix of the tile's digit counts
+  uint32_t glob[kBins];      // where the tile's run of each digit starts in the output
+  uint32_t run[kBins];       // items of each digit ranked so far in this tile
+  uint32_t wcnt[2][kWaves][kBins];  // per-round, per-wave digit counts, tagged with the round (double-buffered)
+  uint32_t w[kWaves + 1];
+};
+
+// offs: the scanned histogram (kBins * nblocks entries) followed by the total.
+__global__ __launch_bounds__(kT) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                                     uint32_t* vout, int64_t n, int shift, const uint32_t* offs,
+                                                     uint32_t nblocks) {
+  RCM_LDS(ScatterSmem, sm);
+  const int t = threadIdx.x;
+  const int w = wave_id();
+  const int lane = lane_id();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t tile_n64 = n - base < kTile ? n - base : kTile;
+  const int tile_n = (int)tile_n64;
+  {
+    const size_t idx = (size_t)t * nblocks + blockIdx.x;
+    const uint32_t start = offs[idx];
+    const uint32_t count = offs[idx + 1] - start;
+    sm.glob[t] = start;
+    sm.run[t] = 0;
+    for (int q = t; q < 2 * kWaves * kBins; q += kT) (&sm.wcnt[0][0][0])[q] = 0;
+    uint32_t tot;
+    sm.tile_off[t] = rcm::block_excl_scan<kT>(count, sm.w, &tot);  // barriers inside; tile_off read later
+  }
+  __syncthreads();
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = base + (int64_t)j * kT + t;
+    const bool valid = i < n;
+    const uint64_t key = valid ? kin[i] : 0ull;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t d = valid ? ((uint32_t)(key >> shift) & (kBins - 1)) : (uint32_t)kBins;
+    // lanes of this wave holding the same digit (9-bit match: kBins marks an empty slot)
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int bit = 0; bit < 9; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t m = ballot(on);
+      peers &= on ? m : ~m;
+    }
+    const uint32_t lower = (uint32_t)__popcll(peers & lanemask_lt());
+    const uint32_t tag = (uint32_t)(j + 1) << 16;
+    const int buf = j & 1;
+    if (valid && (peers >> lane) == 1ull) sm.wcnt[buf][w][d] = tag | (lower + 1);  // highest lane of its group
+    __syncthreads();
+    if (valid) {
+      uint32_t r = sm.run[d] + lower;
+      for (int q = 0; q < w; ++q) {
+        const uint32_t c = sm.wcnt[buf][q][d];
+        r += (c >> 16) == (uint32_t)(j + 1) ? (c & 0xFFFFu) : 0u;
+      }
+      const uint32_t pos = sm.tile_off[d] + r;
+      sm.key[pos] = key;
+      sm.val[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int q = 0; q < kWaves; ++q) {
+      const uint32_t c = sm.wcnt[buf][q][t];
+      add += (c >> 16) == (uint32_t)(j + 1) ? (c & 0xFFFFu) : 0u;
+    }
+    sm.run[t] += add;  // read again only after the next round's first barrier
+  }
+  __syncthreads();
+  for (int j = 0; j < kItems; ++j) {
+    const int pos = j * kT + t;
+    if (pos < tile_n) {
+      const uint64_t key = sm.key[pos];
+      const uint32_t d = (uint32_t)(key >> shift) & (kBins - 1);
+      const uint32_t dst = sm.glob[d] + ((uint32_t)pos - sm.tile_off[d]);
+      kout[dst] = key;
+      vout[dst] = sm.val[pos];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Small element-wise kernels
+// ------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_fill_u32(uint32_t* p, int64_t m, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < m) p[i] = v;
+}
+
+// CSR well-formedness: rp[0] == 0, rp[n] == nnz, monotone rows, columns in [0, cols). flag[0] = 1 if not.
+__global__ __launch_bounds__(kT) void k_check_csr(const int32_t* rp, const int32_t* ci, int32_t n, int64_t nnz,
+                                                 int32_t cols, uint32_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) {
+    const int32_t a = rp[i], b = rp[i + 1];
+    if (a > b || a < 0 || (int64_t)b > nnz) *flag = 1u;
+  }
+  if (i == 0 && (rp[0] != 0 || (int64_t)rp[n] != nnz)) *flag = 1u;
+  if (i < nnz) {
+    const int32_t c = ci[i];
+    if (c < 0 || c >= cols) *flag = 1u;
+  }
+}
+
+// inv[perm[i]] = i; flag[0] = 1 if perm is not a permutation of [0, n) (second kernel confirms bijectivity).
+__global__ __launch_bounds__(kT) void k_perm_inverse(const int32_t* perm, int32_t n, int32_t* inv, uint32_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) {
+    const int32_t p = perm[i];
+    if (p < 0 || p >= n)
+      *flag = 1u;
+    else
+      inv[p] = (int32_t)i;
+  }
+}
+__global__ __launch_bounds__(kT) void k_perm_confirm(const int32_t* perm, int32_t n, const int32_t* inv,
+                                                    uint32_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n && inv[perm[i]] != (int32_t)i) *flag = 1u;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// B = P A P^T: row i of B is row perm[i] of A with column c renamed inv[c], columns sorted, entries with
+// equal new column in their order in A.
+// ------------------------------------------------------------------------------------------------------
+// len[i] (to be scanned into B's row offsets), long_len[i] and long_cnt[i] for rows above kShortRow.
+__global__ __launch_bounds__(kT) void k_perm_lengths(const int32_t* rp, const int32_t* perm, int32_t n,
+                                                    uint32_t* len, uint32_t* long_len, uint32_t* long_cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) {
+    const int32_t r = perm[i];
+    const uint32_t l = (uint32_t)(rp[r + 1] - rp[r]);
+    len[i] = l;
+    long_len[i] = l > (uint32_t)kShortRow ? l : 0u;
+    long_cnt[i] = l > (uint32_t)kShortRow ? 1u : 0u;
+  } else if (i == n) {
+    len[n] = long_len[n] = long_cnt[n] = 0u;
+  }
+}
+
+struct ShortSmem {
+  uint32_t loff[kRowsPerGroup + 1];
+  int32_t src[kRowsPerGroup];
+  int32_t dst[kRowsPerGroup];
+  uint32_t nc[kRowsPerGroup * kShortRow];
+  int32_t k[kRowsPerGroup * kShortRow];
+  uint8_t row[kRowsPerGroup * kShortRow];
+};
+
+template <class V>
+__global__ __launch_bounds__(kT) void k_perm_short(const int32_t* rp, const int32_t* ci, const V* vals,
+                                                  const int32_t* perm, const int32_t* inv, const int32_t* rpB,
+                                                  int32_t n, int32_t* ci_out, V* vals_out) {
+  RCM_LDS(ShortSmem, sm);
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerGroup;
+  if (t < kWave) {  // wave 0: one row per lane
+    const int64_t i = r0 + t;
+    uint32_t l = 0;
+    int32_t src = 0, dst = 0;
+    if (i < n) {
+      src = rp[perm[i]];
+      dst = rpB[i];
+      l = (uint32_t)(rpB[i + 1] - rpB[i]);
+      if (l > (uint32_t)kShortRow) l = 0;  // long rows go through the sort
+    }
+    const uint32_t inc = rcm::wave_incl_scan(l);
+    sm.loff[t + 1] = inc;
+    if (t == 0) sm.loff[0] = 0;
+    sm.src[t] = src;
+    sm.dst[t] = dst;
+  }
+  __syncthreads();
+  const uint32_t total = sm.loff[kRowsPerGroup];
+  for (uint32_t q = t; q < total; q += kT) {
+    int lo = 0, hi = kRowsPerGroup;  // find the row r with loff[r] <= q < loff[r + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sm.loff[mid] <= q)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int32_t k = sm.src[lo] + (int32_t)(q - sm.loff[lo]);
+    sm.nc[q] = (uint32_t)inv[ci[k]];
+    sm.k[q] = k;
+    sm.row[q] = (uint8_t)lo;
+  }
+  __syncthreads();
+  for (uint32_t q = t; q < total; q += kT) {
+    const int r = sm.row[q];
+    const uint32_t a = sm.loff[r], b = sm.loff[r + 1];
+    const uint32_t my = sm.nc[q];
+    uint32_t rank = 0;
+    for (uint32_t f = a; f < b; ++f) {
+      const uint32_t c = sm.nc[f];
+      rank += (c < my || (c == my && f < q)) ? 1u : 0u;
+    }
+    const int32_t dst = sm.dst[r] + (int32_t)rank;
+    ci_out[dst] = (int32_t)my;
+    if (vals) vals_out[dst] = vals[sm.k[q]];
+  }
+}
+
+// list[ordinal] = row, for rows longer than kShortRow (long_ord: scanned long_cnt)
+__global__ __launch_bounds__(kT) void k_long_list(const int32_t* rpB, const uint32_t* long_ord, int32_t n,
+                                                 int32_t* list) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n && rpB[i + 1] - rpB[i] > kShortRow) list[long_ord[i]] = (int32_t)i;
+}
+
+// One workgroup per long row: key = (ordinal << 32) | new column, payload = source entry index.
+__global__ __launch_bounds__(kT) void k_perm_long_gather(const int32_t* rp, const int32_t* ci, const int32_t* perm,
+                                                        const int32_t* inv, const int32_t* rpB,
+                                                        const uint32_t* long_off, const int32_t* list,
+                                                        uint64_t* keys, uint32_t* kidx) {
+  const uint32_t o = blockIdx.x;
+  const int32_t i = list[o];
+  const int32_t src = rp[perm[i]];
+  const int32_t l = rpB[i + 1] - rpB[i];
+  const uint32_t out = long_off[i];
+  for (int32_t j = threadIdx.x; j < l; j += kT) {
+    const int32_t k = src + j;
+    keys[out + (uint32_t)j] = ((uint64_t)o << 32) | (uint32_t)inv[ci[k]];
+    kidx[out + (uint32_t)j] = (uint32_t)k;
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kT) void k_perm_long_scatter(const uint64_t* keys, const uint32_t* kidx, int64_t m,
+                                                         const int32_t* list, const uint32_t* long_off,
+                                                         const int32_t* rpB, const V* vals, int32_t* ci_out,
+                                                         V* vals_out) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= m) return;
+  const uint64_t key = keys[q];
+  const int32_t i = list[(uint32_t)(key >> 32)];
+  const int64_t dst = (int64_t)rpB[i] + (q - (int64_t)long_off[i]);
+  ci_out[dst] = (int32_t)(uint32_t)key;
+  if (vals) vals_out[dst] = vals[kidx[q]];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// COO -> CSR (stable on (row, column)), symmetrisation and connectivity repair.
+// ------------------------------------------------------------------------------------------------------
+// keys[k] = (row << 32) | col, idx[k] = k, cnt[row]++; flag[0] = 1 for an index out of range.
+__global__ __launch_bounds__(kT) void k_coo_keys(const int32_t* r, const int32_t* c, int64_t nnz, int32_t rows,
+                                                int32_t cols, uint64_t* keys, uint32_t* idx, uint32_t* cnt,
+                                                uint32_t* flag) {
+  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (k >= nnz) return;
+  const int32_t a = r[k], b = c[k];
+  if (a < 0 || a >= rows || b < 0 || b >= cols) {
+    *flag = 1u;
+    keys[k] = 0;
+  } else {
+    keys[k] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+    atomicAdd(&cnt[a], 1u);
+  }
+  idx[k] = (uint32_t)k;
+}
+
+template <class V>
+__global__ __launch_bounds__(kT) void k_csr_from_sorted(const uint64_t* keys, const uint32_t* idx, int64_t nnz,
+                                                       const V* vin, int32_t* ci_out, V* vout) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz) return;
+  ci_out[q] = (int32_t)(uint32_t)keys[q];
+  if (vin) vout[q] = vin[idx[q]];
+}
+
+// off-diagonal flags of the CSR's entries (rows from the sorted keys)
+__global__ __launch_bounds__(kT) void k_offdiag_flags(const uint64_t* keys, int64_t nnz, uint32_t* flags) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz) return;
+  const uint64_t key = keys[q];
+  flags[q] = (uint32_t)(key >> 32) != (uint32_t)key ? 1u : 0u;
+}
+
+// unordered pair key (min << 32 | max) of each off-diagonal entry, compacted by the scanned flags
+__global__ __launch_bounds__(kT) void k_pair_keys(const uint64_t* keys, int64_t nnz, const uint32_t* pos,
+                                                 uint64_t* pk, uint32_t* pidx) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz) return;
+  const uint64_t key = keys[q];
+  const uint32_t r = (uint32_t)(key >> 32), c = (uint32_t)key;
+  if (r == c) return;
+  const uint32_t lo = r < c ? r : c, hi = r < c ? c : r;
+  pk[pos[q]] = ((uint64_t)lo << 32) | hi;
+  pidx[pos[q]] = (uint32_t)q;
+}
+
+// odd[entry] = 1 for every entry of a pair stored an odd number of times (runs of the sorted pair keys)
+__global__ __launch_bounds__(kT) void k_pair_odd(const uint64_t* pk, const uint32_t* pidx, int64_t m, uint32_t* odd) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= m) return;
+  if (s > 0 && pk[s - 1] == pk[s]) return;  // not the start of a run
+  int64_t e = s + 1;
+  while (e < m && pk[e] == pk[s]) ++e;
+  if (((e - s) & 1) == 0) return;
+  for (int64_t x = s; x < e; ++x) odd[pidx[x]] = 1u;
+}
+
+// COO in the insertion order of the symmetrisation: each entry, then its transpose when its pair is odd.
+// opos: scanned odd flags (nnz + 1 entries).
+template <class V>
+__global__ __launch_bounds__(kT) void k_sym_expand(const uint64_t* keys, const V* vals, const uint32_t* opos,
+                                                  int64_t nnz, int32_t* r_out, int32_t* c_out, V* v_out) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz) return;
+  const uint64_t key = keys[q];
+  const int32_t r = (int32_t)(key >> 32), c = (int32_t)(uint32_t)key;
+  const int64_t p = q + (int64_t)opos[q];
+  r_out[p] = r;
+  c_out[p] = c;
+  if (vals) v_out[p] = vals[q];
+  if (opos[q + 1] != opos[q]) {
+    r_out[p + 1] = c;
+    c_out[p + 1] = r;
+    if (vals) v_out[p + 1] = vals[q];
+  }
+}
+
+// flag[0] = 1: an empty row; flag[1] = 1: an entry (r, c) without (c, r) (columns are sorted)
+__global__ __launch_bounds__(kT) void k_struct_checks(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* flag) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r >= n) return;
+  const int32_t a = rp[r], b = rp[r + 1];
+  if (a == b) flag[0] = 1u;
+  for (int32_t k = a; k < b; ++k) {
+    const int32_t c = ci[k];
+    int32_t lo = rp[c], hi = rp[c + 1];  // lower bound of r in row c
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (ci[mid] < (int32_t)r)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo == rp[c + 1] || ci[lo] != (int32_t)r) {
+      flag[1] = 1u;
+      return;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_cc_init(uint32_t* parent, int32_t n) {
+  const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) parent[v] = (uint32_t)v;
+}
+
+// Root of x with path halving. Parents only ever move to smaller indices, so every chain ends.
+__device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t x) {
+  uint32_t p = ld_u32(parent + x);
+  while (p != x) {
+    const uint32_t g = ld_u32(parent + p);
+    if (g != p) st_u32(parent + x, g);
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
+// Concurrent union-find over the edges (u, v), v < u: the larger root is hooked under the smaller one with a
+// compare-and-swap, so each tree's root is its smallest node.
+__global__ __launch_bounds__(kT) void k_cc_hook(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
+  const int64_t u = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (u >= n) return;
+  for (int32_t k = rp[u]; k < rp[u + 1]; ++k) {
+    const uint32_t v = (uint32_t)ci[k];
+    if (v >= (uint32_t)u) break;  // sorted columns: each undirected edge once
+    uint32_t a = cc_find(parent, (uint32_t)u), b = cc_find(parent, v);
+    while (a != b) {
+      const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+      uint32_t expected = hi;
+      if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+        break;
+      a = cc_find(parent, hi);
+      b = cc_find(parent, lo);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_cc_finish(uint32_t* parent, int32_t n) {
+  const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) st_u32(parent + v, cc_find(parent, (uint32_t)v));
+}
+
+// New row lengths with the links (root - 1, root) of every component other than node 0's.
+__global__ __launch_bounds__(kT) void k_link_lengths(const int32_t* rp, const uint32_t* root, int32_t n,
+                                                    uint32_t* len) {
+  const int64_t x = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (x < n) {
+    uint32_t l = (uint32_t)(rp[x + 1] - rp[x]);
+    if (x > 0 && root[x] == (uint32_t)x) ++l;
+    if (x + 1 < n && root[x + 1] == (uint32_t)(x + 1)) ++l;
+    len[x] = l;
+  } else if (x == n) {
+    len[n] = 0;
+  }
+}
+
+// Row x with its link entries merged in column order (value 1): x - 1 if x is a component root, x + 1 if
+// x + 1 is one. Neither column is already present (the nodes lie in different components).
+template <class V>
+__global__ __launch_bounds__(kT) void k_link_insert(const int32_t* rp, const int32_t* ci, const V* vals,
+                                                   const uint32_t* root, int32_t n, const int32_t* rp2,
+                                                   int32_t* ci2, V* vals2) {
+  const int64_t x = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (x >= n) return;
+  int32_t extra[2];
+  int ne = 0;
+  if (x > 0 && root[x] == (uint32_t)x) extra[ne++] = (int32_t)x - 1;
+  if (x + 1 < n && root[x + 1] == (uint32_t)(x + 1)) extra[ne++] = (int32_t)x + 1;
+  int32_t o = rp2[x];
+  int e = 0;
+  for (int32_t k = rp[x]; k < rp[x + 1]; ++k) {
+    const int32_t c = ci[k];
+    while (e < ne && extra[e] < c) {
+      ci2[o] = extra[e++];
+      if (vals) vals2[o] = (V)1;
+      ++o;
+    }
+    ci2[o] = c;
+    if (vals) vals2[o] = vals[k];
+    ++o;
+  }
+  while (e < ne) {
+    ci2[o] = extra[e++];
+    if (vals) vals2[o] = (V)1;
+    ++o;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Host orchestration, generic over the executor X:
+//   template <class K, class... A> void run(K kernel, int64_t grid, A... args);   // kT threads per group
+//   template <class T> T* alloc(size_t count);   // scratch, freed by the executor
+//   uint32_t read_u32(const uint32_t* p);          // synchronous readback
+//   bool failed();                                 // a launch or allocation failed
+// ------------------------------------------------------------------------------------------------------
+inline int64_t groups(int64_t items, int64_t per) { return (items + per - 1) / per; }
+
+template <class X>
+void scan_u32(X& x, const uint32_t* in, uint32_t* out, int64_t m, uint32_t* total_out) {
+  const int64_t tiles = groups(m > 0 ? m : 1, kTile);
+  uint32_t* partial = x.template alloc<uint32_t>((size_t)tiles);
+  if (!partial) return;
+  x.run(k_scan_reduce, tiles, in, m, partial);
+  x.run(k_scan_top, 1, partial, tiles, total_out);
+  x.run(k_scan_apply, tiles, in, out, m, (const uint32_t*)partial);
+}
+
+// Shifts of the 8-bit passes that cover `lo_bits` of the low word and `hi_bits` of the high word.
+inline int radix_shifts(int lo_bits, int hi_bits, int* shifts) {
+  int k = 0;
+  for (int s = 0; s < lo_bits; s += 8) shifts[k++] = s;
+  for (int s = 0; s < hi_bits; s += 8) shifts[k++] = 32 + s;
+  return k;
+}
+
+inline int bits_of(uint64_t maxval) {
+  int b = 0;
+  while (maxval) {
+    ++b;
+    maxval >>= 1;
+  }
+  return b;
+}
+
+// Sorts (keys, vals) stably; the result ends in whichever buffer pair the last pass wrote: returned via the
+// references. alt buffers must hold n items.
+template <class X>
+void radix_sort(X& x, uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt, uint32_t*& vals_alt, int64_t n,
+                int lo_bits, int hi_bits) {
+  if (n <= 1) return;
+  int shifts[8];
+  const int passes = radix_shifts(lo_bits, hi_bits, shifts);
+  const int64_t tiles = groups(n, kTile);
+  const int64_t m = (int64_t)kBins * tiles;
+  uint32_t* hist = x.template alloc<uint32_t>((size_t)m + 1);
+  if (!hist) return;
+  for (int p = 0; p < passes; ++p) {
+    x.run(k_radix_hist, tiles, (const uint64_t*)keys, n, shifts[p], hist, (uint32_t)tiles);
+    scan_u32(x, hist, hist, m, hist + m);
+    x.run(k_radix_scatter, tiles, (const uint64_t*)keys, (const uint32_t*)vals, keys_alt, vals_alt, n, shifts[p],
+          (const uint32_t*)hist, (uint32_t)tiles);
+    uint64_t* tk = keys;
+    keys = keys_alt;
+    keys_alt = tk;
+    uint32_t* tv = vals;
+    vals = vals_alt;
+    vals_alt = tv;
+  }
+}
+
+enum OpsStatus { kOpsOk = 0, kOpsInvalid = -1, kOpsDevice = -4, kOpsNotSymmetric = -12, kOpsEmptyRow = -13 };
+
+// B = P A P^T. rp_out: n + 1, ci_out / vals_out: nnz. vals may be null.
+template <class X, class V>
+int permute(X& x, int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const V* vals, const int32_t* perm,
+            int32_t* rp_out, int32_t* ci_out, V* vals_out) {
+  uint32_t* flag = x.template alloc<uint32_t>(1);
+  int32_t* inv = x.template alloc<int32_t>((size_t)n);
+  uint32_t* long_len = x.template alloc<uint32_t>((size_t)n + 1);
+  uint32_t* long_cnt = x.template alloc<uint32_t>((size_t)n + 1);
+  uint32_t* tot = x.template alloc<uint32_t>(2);
+  if (x.failed()) return kOpsDevice;
+  x.run(k_fill_u32, 1, flag, (int64_t)1, 0u);
+  x.run(k_check_csr, groups(nnz > n ? nnz : (int64_t)n, kT), rp, ci, n, nnz, n, flag);
+  x.run(k_perm_inverse, groups(n, kT), perm, n, inv, flag);
+  if (x.read_u32(flag)) return kOpsInvalid;  // perm out of range or malformed CSR
+  x.run(k_perm_confirm, groups(n, kT), perm, n, (const int32_t*)inv, flag);
+  if (x.read_u32(flag)) return kOpsInvalid;  // perm repeats a node
+  uint32_t* len = reinterpret_cast<uint32_t*>(rp_out);
+  x.run(k_perm_lengths, groups((int64_t)n + 1, kT), rp, perm, n, len, long_len, long_cnt);
+  scan_u32(x, len, len, (int64_t)n + 1, (uint32_t*)nullptr);
+  scan_u32(x, long_len, long_len, (int64_t)n + 1, tot);
+  scan_u32(x, long_cnt, long_cnt, (int64_t)n + 1, tot + 1);
+  x.run(k_perm_short<V>, groups(n, kRowsPerGroup), rp, ci, vals, perm, (const int32_t*)inv, (const int32_t*)rp_out,
+        n, ci_out, vals_out);
+  const uint32_t long_entries = x.read_u32(tot);
+  const uint32_t long_rows = x.read_u32(tot + 1);
+  if (x.failed()) return kOpsDevice;
+  if (long_rows == 0) return kOpsOk;
+  int32_t* list = x.template alloc<int32_t>(long_rows);
+  uint64_t* keys = x.template alloc<uint64_t>(long_entries);
+  uint64_t* keys2 = x.template alloc<uint64_t>(long_entries);
+  uint32_t* kidx = x.template alloc<uint32_t>(long_entries);
+  uint32_t* kidx2 = x.template alloc<uint32_t>(long_entries);
+  if (x.failed()) return kOpsDevice;
+  x.run(k_long_list, groups(n, kT), (const int32_t*)rp_out, (const uint32_t*)long_cnt, n, list);
+  x.run(k_perm_long_gather, (int64_t)long_rows, rp, ci, perm, (const int32_t*)inv, (const int32_t*)rp_out,
+        (const uint32_t*)long_len, (const int32_t*)list, keys, kidx);
+  radix_sort(x, keys, kidx, keys2, kidx2, (int64_t)long_entries, bits_of((uint64_t)(n - 1)),
+             bits_of((uint64_t)long_rows - 1));
+  x.run(k_perm_long_scatter<V>, groups(long_entries, kT), (const uint64_t*)keys, (const uint32_t*)kidx,
+        (int64_t)long_entries, (const int32_t*)list, (const uint32_t*)long_len, (const int32_t*)rp_out, vals,
+        ci_out, vals_out);
+  return x.failed() ? kOpsDevice : kOpsOk;
+}
+
+// Output CSR of canonicalise(); device buffers allocated through the executor's `keep` allocator.
+template <class V>
+struct Csr {
+  int32_t rows = 0, cols = 0;
+  int64_t nnz = 0;
+  int32_t* rp = nullptr;
+  int32_t* ci = nullptr;
+  V* vals = nullptr;
+};
+
+// COO -> CSR, stable on (row, column).
+template <class X, class V>
+int coo_to_csr(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, const int32_t* c, const V* vals,
+               Csr<V>& out, uint64_t** sorted_keys) {
+  uint32_t* flag = x.template alloc<uint32_t>(1);
+  uint64_t* keys = x.template alloc<uint64_t>((size_t)nnz);
+  uint64_t* keys2 = x.template alloc<uint64_t>((size_t)nnz);
+  uint32_t* idx = x.template alloc<uint32_t>((size_t)nnz);
+  uint32_t* idx2 = x.template alloc<uint32_t>((size_t)nnz);
+  out.rows = rows;
+  out.cols = cols;
+  out.nnz = nnz;
+  out.rp = x.template keep<int32_t>((size_t)rows + 1);
+  out.ci = x.template keep<int32_t>((size_t)nnz);
+  out.vals = vals ? x.template keep<V>((size_t)nnz) : nullptr;
+  if (x.failed()) return kOpsDevice;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(out.rp);
+  x.run(k_fill_u32, groups((int64_t)rows + 1, kT), cnt, (int64_t)rows + 1, 0u);
+  x.run(k_fill_u32, 1, flag, (int64_t)1, 0u);
+  if (nnz > 0) x.run(k_coo_keys, groups(nnz, kT), r, c, nnz, rows, cols, keys, idx, cnt, flag);
+  if (x.read_u32(flag)) return kOpsInvalid;
+  scan_u32(x, cnt, cnt, (int64_t)rows + 1, (uint32_t*)nullptr);
+  radix_sort(x, keys, idx, keys2, idx2, nnz, bits_of(cols > 0 ? (uint64_t)(cols - 1) : 0),
+             bits_of(rows > 0 ? (uint64_t)(rows - 1) : 0));
+  if (nnz > 0) x.run(k_csr_from_sorted<V>, groups(nnz, kT), (const uint64_t*)keys, (const uint32_t*)idx, nnz, vals,
+                     out.ci, out.vals);
+  if (sorted_keys) *sorted_keys = keys;
+  return x.failed() ? kOpsDevice : kOpsOk;
+}
+
+// convert + prepareRCM(check_sym) (src/CSR.cpp:170-208, include/Graph.h:74-264) for structurally symmetric
+// results: the connectivity step needs (r, c) <=> (c, r) and no empty row (which the reference's repair loop
+// never leaves). *links_out: links added.
+template <class X, class V>
+int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, const int32_t* c, const V* vals,
+                 bool check_sym, Csr<V>& out, int64_t* links_out) {
+  Csr<V> a;
+  uint64_t* skeys = nullptr;
+  int rc = coo_to_csr(x, rows, cols, nnz, r, c, vals, a, &skeys);
+  if (rc) return rc;
+  if (check_sym && nnz > 0) {
+    uint32_t* pos = x.template alloc<uint32_t>((size_t)nnz + 1);
+    uint32_t* tot = x.template alloc<uint32_t>(2);
+    if (x.failed()) return kOpsDevice;
+    x.run(k_offdiag_flags, groups(nnz, kT), (const uint64_t*)skeys, nnz, pos);
+    x.run(k_fill_u32, 1, pos + nnz, (int64_t)1, 0u);
+    scan_u32(x, pos, pos, nnz + 1, tot);
+    const uint32_t m = x.read_u32(tot);
+    uint32_t* odd = x.template alloc<uint32_t>((size_t)nnz + 1);
+    if (x.failed()) return kOpsDevice;
+    x.run(k_fill_u32, groups(nnz + 1, kT), odd, nnz + 1, 0u);
+    if (m > 0) {
+      uint64_t* pk = x.template alloc<uint64_t>(m);
+      uint64_t* pk2 = x.template alloc<uint64_t>(m);
+      uint32_t* pidx = x.template alloc<uint32_t>(m);
+      uint32_t* pidx2 = x.template alloc<uint32_t>(m);
+      if (x.failed()) return kOpsDevice;
+      x.run(k_pair_keys, groups(nnz, kT), (const uint64_t*)skeys, nnz, (const uint32_t*)pos, pk, pidx);
+      const int nb = bits_of((uint64_t)(rows > cols ? rows : cols));
+      radix_sort(x, pk, pidx, pk2, pidx2, (int64_t)m, nb, nb);
+      x.run(k_pair_odd, groups(m, kT), (const uint64_t*)pk, (const uint32_t*)pidx, (int64_t)m, odd);
+    }
+    scan_u32(x, odd, odd, nnz + 1, tot + 1);
+    const uint32_t extra = x.read_u32(tot + 1);
+    if (x.failed()) return kOpsDevice;
+    if (extra > 0) {
+      const int64_t nnz2 = nnz + (int64_t)extra;
+      if (nnz2 > INT32_MAX) return kOpsInvalid;
+      int32_t* r2 = x.template alloc<int32_t>((size_t)nnz2);
+      int32_t* c2 = x.template alloc<int32_t>((size_t)nnz2);
+      V* v2 = vals ? x.template alloc<V>((size_t)nnz2) : nullptr;
+      if (x.failed()) return kOpsDevice;
+      x.run(k_sym_expand<V>, groups(nnz, kT), (const uint64_t*)skeys, (const V*)a.vals, (const uint32_t*)odd, nnz,
+            r2, c2, v2);
+      const int32_t s = rows > cols ? rows : cols;
+      Csr<V> b;
+      rc = coo_to_csr(x, s, s, nnz2, r2, c2, (const V*)v2, b, (uint64_t**)nullptr);
+      if (rc) return rc;
+      a = b;
+    }
+  }
+  if (a.rows != a.cols) return kOpsInvalid;
+  const int32_t n = a.rows;
+  if (n < 1) return kOpsInvalid;
+  uint32_t* flag = x.template alloc<uint32_t>(2);
+  uint32_t* root = x.template alloc<uint32_t>((size_t)n);
+  if (x.failed()) return kOpsDevice;
+  x.run(k_fill_u32, 1, flag, (int64_t)2, 0u);
+  x.run(k_struct_checks, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, flag);
+  if (x.read_u32(flag + 1)) return kOpsNotSymmetric;
+  if (x.read_u32(flag)) return kOpsEmptyRow;
+  x.run(k_cc_init, groups(n, kT), root, n);
+  x.run(k_cc_hook, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, root);
+  x.run(k_cc_finish, groups(n, kT), root, n);
+  uint32_t* tot = x.template alloc<uint32_t>(1);
+  int32_t* rp2 = x.template keep<int32_t>((size_t)n + 1);
+  if (x.failed()) return kOpsDevice;
+  uint32_t* len = reinterpret_cast<uint32_t*>(rp2);
+  x.run(k_link_lengths, groups((int64_t)n + 1, kT), (const int32_t*)a.rp, (const uint32_t*)root, n, len);
+  scan_u32(x, len, len, (int64_t)n + 1, tot);
+  const int64_t nnz2 = (int64_t)x.read_u32(tot);
+  if (x.failed()) return kOpsDevice;
+  const int64_t links = (nnz2 - a.nnz) / 2;
+  *links_out = links;
+  if (links == 0) {
+    out = a;
+    return kOpsOk;
+  }
+  Csr<V> b;
+  b.rows = b.cols = n;
+  b.nnz = nnz2;
+  b.rp = rp2;
+  b.ci = x.template keep<int32_t>((size_t)nnz2);
+  b.vals = vals ? x.template keep<V>((size_t)nnz2) : nullptr;
+  if (x.failed()) return kOpsDevice;
+  x.run(k_link_insert<V>, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, (const V*)a.vals,
+        (const uint32_t*)root, n, (const int32_t*)rp2, b.ci, b.vals);
+  out = b;
+  return x.failed() ? kOpsDevice : kOpsOk;
+}
+
+}  // namespace rcm_mx

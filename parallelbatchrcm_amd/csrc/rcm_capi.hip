@@ -686,6 +686,7 @@ const char* rcm_status_string(int s) {
     case RCM_ENOTFOUND: return "unknown request id";
     case RCM_EAGAIN: return "request not finished yet";
     case RCM_EIO: return "matrix file could not be opened, read or written";
+    case RCM_ENOTSYMMETRIC: return "matrix is not structurally symmetric";
     default: return "unknown status";
   }
 }

@@ -21,7 +21,7 @@ def lib(engine: str = "block"):
     if engine not in _libs:
         subprocess.run(["make", "-s", "-C", EMU_DIR], check=True)
         L = C.CDLL(os.path.join(EMU_DIR, f"libemu_{engine}.so"))
-        if engine == "many":
+        if engine in ("many", "ops"):
             _libs[engine] = L
             return L
         L.emu_run.restype = C.c_int
@@ -83,3 +83,75 @@ def run_many(mats, starts, workers=2, bfs=False, timeout_s=30.0):
     perms = [perm[offs[k]:offs[k + 1]] for k in range(len(mats))]
     levs = [lev[offs[k]:offs[k + 1]] for k in range(len(mats))]
     return perms, levs, info, done
+
+
+# ---- matrix_ops.hpp kernels (canonicalisation, permutation, radix sort, scan) ----------------------------
+def _ops_lib():
+    L = lib("ops")
+    if not hasattr(L, "_ops_ready"):
+        I64P = C.POINTER(C.c_int64)
+        DP = C.POINTER(C.c_double)
+        U64P = C.POINTER(C.c_uint64)
+        U32P = C.POINTER(C.c_uint32)
+        L.emu_permute.restype = C.c_int
+        L.emu_permute.argtypes = [C.c_int32, C.c_int64, I32P, I32P, DP, I32P, I32P, I32P, DP]
+        L.emu_canonicalize.restype = C.c_int
+        L.emu_canonicalize.argtypes = [C.c_int32, C.c_int32, C.c_int64, I32P, I32P, DP, C.c_int32, C.c_int64, I32P,
+                                       I64P, I64P, I32P, I32P, DP]
+        L.emu_radix_sort.restype = C.c_int
+        L.emu_radix_sort.argtypes = [C.c_int64, U64P, U32P, C.c_int32, C.c_int32]
+        L.emu_scan.restype = C.c_int
+        L.emu_scan.argtypes = [C.c_int64, U32P, U32P]
+        L._ops_ready = True
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def scan(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty(a.size + 1, np.uint32)
+    _ops_lib().emu_scan(a.size, a.ctypes.data_as(C.POINTER(C.c_uint32)), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def radix_sort(keys, vals, lo_bits, hi_bits):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    _ops_lib().emu_radix_sort(k.size, k.ctypes.data_as(C.POINTER(C.c_uint64)), v.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              lo_bits, hi_bits)
+    return k, v
+
+
+def permute(rp, ci, vals, perm):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    perm = np.ascontiguousarray(perm, np.int32)
+    vals = None if vals is None else np.ascontiguousarray(vals, np.float64)
+    n, nnz = rp.size - 1, ci.size
+    rp2 = np.empty(n + 1, np.int32)
+    ci2 = np.empty(max(nnz, 1), np.int32)
+    v2 = None if vals is None else np.empty(max(nnz, 1), np.float64)
+    rc = _ops_lib().emu_permute(n, nnz, rp.ctypes.data_as(I32P), ci.ctypes.data_as(I32P), _dp(vals),
+                                perm.ctypes.data_as(I32P), rp2.ctypes.data_as(I32P), ci2.ctypes.data_as(I32P), _dp(v2))
+    return rc, rp2, ci2[:nnz], (None if v2 is None else v2[:nnz])
+
+
+def canonicalize(rows, cols, r, c, vals, check_sym):
+    r = np.ascontiguousarray(r, np.int32)
+    c = np.ascontiguousarray(c, np.int32)
+    vals = None if vals is None else np.ascontiguousarray(vals, np.float64)
+    cap = 2 * r.size + 2 * max(rows, cols) + 16
+    rp = np.empty(max(rows, cols) + 1, np.int32)
+    ci = np.empty(cap, np.int32)
+    v = None if vals is None else np.empty(cap, np.float64)
+    n, nnz, links = C.c_int32(), C.c_int64(), C.c_int64()
+    rc = _ops_lib().emu_canonicalize(rows, cols, r.size, r.ctypes.data_as(I32P), c.ctypes.data_as(I32P), _dp(vals),
+                                     1 if check_sym else 0, cap, C.byref(n), C.byref(nnz), C.byref(links),
+                                     rp.ctypes.data_as(I32P), ci.ctypes.data_as(I32P), _dp(v))
+    if rc:
+        return rc, None
+    m = nnz.value
+    return 0, (n.value, rp[: n.value + 1].copy(), ci[:m].copy(), None if v is None else v[:m].copy(), links.value)

@@ -1,0 +1,113 @@
+"""CPU checks of the canonicalisation and permutation kernels (parallelbatchrcm_amd/csrc/matrix_ops.hpp) run in
+the wave emulator through the library's own orchestration: every lane is a host thread, ballots and barriers are
+collectives. Checked against numpy definitions and the host C++ convert / prepareRCM, which tests/test_formats.py
+pins to the reference's own code."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import emu_lib as E
+import matrix_ref as M
+import parallelbatchrcm_amd as P
+
+
+def test_scan_closed_form():
+    for m in (1, 255, 4096, 4097, 9000):
+        a = np.random.default_rng(m).integers(0, 1000, m).astype(np.uint32)
+        np.testing.assert_array_equal(E.scan(a), np.concatenate([[0], np.cumsum(a)]).astype(np.uint32))
+
+
+@pytest.mark.parametrize("lo_bits,hi_bits", [(8, 0), (12, 6), (3, 17), (24, 0)])
+def test_radix_sort_is_stable(lo_bits, hi_bits):
+    rng = np.random.default_rng(lo_bits * 31 + hi_bits)
+    n = 7000
+    lo = rng.integers(0, 1 << lo_bits, n).astype(np.uint64) if lo_bits else np.zeros(n, np.uint64)
+    hi = rng.integers(0, 1 << hi_bits, n).astype(np.uint64) if hi_bits else np.zeros(n, np.uint64)
+    lo[::7] = lo[0]  # many ties
+    keys = (hi << np.uint64(32)) | lo
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = E.radix_sort(keys, vals, lo_bits, hi_bits)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])  # equal keys keep their input order
+
+
+def matrix_with_long_rows(rng, n, k):
+    r = rng.integers(0, n, k)
+    c = rng.integers(0, n, k)
+    # two long rows (> 64 entries, one with repeated columns) and a long column
+    r = np.concatenate([r, np.full(300, 5), np.full(90, 11), rng.integers(0, n, 150)])
+    c = np.concatenate([c, rng.integers(0, n, 300), rng.integers(0, 7, 90), np.full(150, 9)])
+    return P.convert_coo(n, n, r, c, rng.integers(-99, 99, r.size).astype(np.float64))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_permute_matches_definition(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(300, 900))
+    m = matrix_with_long_rows(rng, n, 5 * n)
+    perm = rng.permutation(n).astype(np.int32)
+    rc, rp2, ci2, v2 = E.permute(m.row_offsets, m.col_ids, m.data, perm)
+    assert rc == 0
+    erp, eci, ev = M.permute_ref(m.row_offsets, m.col_ids, m.data, perm)
+    np.testing.assert_array_equal(rp2, erp)
+    np.testing.assert_array_equal(ci2, eci)
+    np.testing.assert_array_equal(v2, ev)
+    # dense form: B = A[perm][:, perm]
+    np.testing.assert_array_equal(M.dense(rp2, ci2, v2, n), M.dense(m.row_offsets, m.col_ids, m.data, n)[perm][:, perm])
+
+
+def test_permute_rejects_bad_input():
+    rng = np.random.default_rng(7)
+    m = matrix_with_long_rows(rng, 200, 600)
+    perm = rng.permutation(200).astype(np.int32)
+    bad = perm.copy()
+    bad[3] = bad[4]
+    assert E.permute(m.row_offsets, m.col_ids, m.data, bad)[0] == -1
+    bad = perm.copy()
+    bad[0] = 200
+    assert E.permute(m.row_offsets, m.col_ids, m.data, bad)[0] == -1
+    rp = m.row_offsets.copy()
+    rp[-1] += 1  # row offsets do not span nnz
+    assert E.permute(rp, m.col_ids, m.data, perm)[0] == -1
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("check_sym", [True, False])
+def test_canonicalize_matches_host(seed, check_sym):
+    rng = np.random.default_rng(50 + seed)
+    n = int(rng.integers(60, 500))
+    r, c = M.coo_with_components(rng, n, 4 * n, comps=int(rng.integers(1, 6)))
+    v = rng.integers(-99, 99, r.size).astype(np.float64)
+    if not check_sym:  # a symmetric input, its components still apart
+        sym = P.convert_coo(n, n, np.concatenate([r, c]), np.concatenate([c, r]), np.concatenate([v, v]))
+        r, c, v = M.rows_of(sym.row_offsets).astype(np.int32), sym.col_ids, sym.data
+    rc, res = E.canonicalize(n, n, r, c, v, check_sym)
+    assert rc == 0
+    nn, rp, ci, vv, links = res
+    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), check_sym)
+    assert links == hlinks
+    np.testing.assert_array_equal(rp, host.row_offsets)
+    np.testing.assert_array_equal(ci, host.col_ids)
+    np.testing.assert_array_equal(vv, host.data)
+
+
+def test_canonicalize_refusals():
+    # a pair stored twice in one direction is "symmetric" to the reference's parity check but not structurally
+    r = np.array([0, 0, 0, 1, 2, 1], np.int32)
+    c = np.array([1, 1, 0, 1, 2, 2], np.int32)
+    rc, _ = E.canonicalize(3, 3, r, c, None, True)
+    assert rc == -12
+    # the host repair follows the directed rows there and still succeeds
+    _, links = P.prepare_rcm(P.convert_coo(3, 3, r, c, None), True)
+    assert links >= 0
+    # an empty row: the reference's repair loop does not terminate
+    rc, _ = E.canonicalize(3, 3, np.array([0, 0, 1], np.int32), np.array([0, 1, 0], np.int32), None, True)
+    assert rc == -13
+    # an index out of range
+    rc, _ = E.canonicalize(3, 3, np.array([0, 3], np.int32), np.array([0, 0], np.int32), None, True)
+    assert rc == -1
+    # non-square with nothing to mirror
+    rc, _ = E.canonicalize(2, 3, np.array([0, 1], np.int32), np.array([0, 1], np.int32), None, False)
+    assert rc == -1
