@@ -207,6 +207,69 @@ def bench_serving(requests: int, rates, seed: int, lanes: int):
     return out
 
 
+def permute_bytes(n: int, nnz: int) -> int:
+    """Compulsory HBM bytes of B = P A P^T with float64 values: read A (row_ptr, col_idx, values) and perm,
+    write B."""
+    return 2 * (4 * (n + 1) + 12 * nnz) + 4 * n
+
+
+def canonicalize_bytes(n: int, nnz_in: int, nnz_out: int) -> int:
+    """Compulsory HBM bytes of COO -> CSR + prepareRCM with float64 values: read (row, col, value) triples,
+    write row_ptr, col_idx and values."""
+    return 16 * nnz_in + 4 * (n + 1) + 12 * nnz_out
+
+
+def bench_pipeline(m, perm, steps: int, seed: int):
+    """The steps either side of the reorder on the 128^3 stencil: canonicalisation of its lower triangle as a
+    shuffled COO with float64 values (a MatrixMarket 'symmetric' file's content), and B = P A P^T of the
+    canonical matrix with the RCM permutation. Device time from HIP events around each call; the host C++
+    convert + prepareRCM on the same COO beside it."""
+    import numpy as np
+
+    import parallelbatchrcm_amd as P
+
+    rng = np.random.default_rng(seed)
+    n = m.rows
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(m.row_offsets))
+    lower = rows >= m.col_ids
+    r = rows[lower]
+    c = m.col_ids[lower]
+    order = rng.permutation(r.size)
+    r, c = np.ascontiguousarray(r[order]), np.ascontiguousarray(c[order])
+    v = rng.standard_normal(r.size)
+    out = {"workload": "128^3 7-point stencil: lower triangle as a shuffled COO with float64 values "
+                       f"({r.size} entries) -> canonical CSR ({m.nnz} entries); then P A P^T with the RCM order"}
+    with P.MatrixOps() as ops:
+        canon, links = ops.canonicalize(n, n, r, c, v, True)  # warm-up
+        cms = []
+        for _ in range(steps):
+            ops.canonicalize(n, n, r, c, v, True)
+            cms.append(ops.last_ms)
+        b = ops.permute(canon, perm)  # warm-up
+        pms = []
+        for _ in range(steps):
+            ops.permute(canon, perm)
+            pms.append(ops.last_ms)
+    t0 = time.perf_counter()
+    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), True)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    same = (hlinks == links and np.array_equal(host.col_ids, canon.col_ids) and np.array_equal(host.data, canon.data))
+    cm = float(np.mean(cms))
+    pm = float(np.mean(pms))
+    cb = canonicalize_bytes(n, r.size, canon.nnz)
+    pb = permute_bytes(n, canon.nnz)
+    out["canonicalize"] = {"device_ms": cm, "device_ms_min": float(np.min(cms)), "alg_bytes": cb,
+                           "achieved_GBs": cb / (cm * 1e-3) / 1e9, "frac": cb / (cm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                           "entries_per_s": r.size / (cm * 1e-3), "links": links,
+                           "host_cpp_ms": host_ms, "equal_to_host": bool(same)}
+    out["permute"] = {"device_ms": pm, "device_ms_min": float(np.min(pms)), "alg_bytes": pb,
+                      "achieved_GBs": pb / (pm * 1e-3) / 1e9, "frac": pb / (pm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "nnz_per_s": canon.nnz / (pm * 1e-3)}
+    print(f"[bench] pipeline: canonicalize {cm:.2f} ms (host C++ {host_ms:.0f} ms), permute {pm:.2f} ms",
+          file=sys.stderr, flush=True)
+    return out
+
+
 COPY_BYTES = 256 << 20  # calibration copy for the HBM traffic counters
 
 
@@ -266,6 +329,8 @@ def main() -> int:
     ap.add_argument("--serving", type=int, default=400, help="serving form: requests per arrival rate (0 = skip)")
     ap.add_argument("--serving-rates", type=str, default="200,1000,4000")
     ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--pipeline", type=int, default=5,
+                    help="steps of the canonicalise and permute legs on the 128^3 stencil (0 = skip)")
     args = ap.parse_args()
     if args.pmc_probe:
         return pmc_probe(args.pmc_probe, args.claim_filter)
@@ -354,6 +419,9 @@ def main() -> int:
                             "candidates": tune.candidates, "best_ms": tune.best_ms, "default_ms": tune.default_ms,
                             "tuning_s": tune_s}
     ctx.close()
+
+    if args.pipeline > 0:
+        line["pipeline"] = bench_pipeline(m, perm, args.pipeline, 9)
 
     if args.batched > 0:
         print(f"[bench] batched form: {args.batched} matrices", file=sys.stderr, flush=True)

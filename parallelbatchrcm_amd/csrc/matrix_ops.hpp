@@ -358,10 +358,9 @@ __global__ __launch_bounds__(kT) void k_perm_long_scatter(const uint64_t* keys, 
 // ------------------------------------------------------------------------------------------------------
 // COO -> CSR (stable on (row, column)), symmetrisation and connectivity repair.
 // ------------------------------------------------------------------------------------------------------
-// keys[k] = (row << 32) | col, idx[k] = k, cnt[row]++; flag[0] = 1 for an index out of range.
+// keys[k] = (row << 32) | col, idx[k] = k; flag[0] = 1 for an index out of range.
 __global__ __launch_bounds__(kT) void k_coo_keys(const int32_t* r, const int32_t* c, int64_t nnz, int32_t rows,
-                                                int32_t cols, uint64_t* keys, uint32_t* idx, uint32_t* cnt,
-                                                uint32_t* flag) {
+                                                int32_t cols, uint64_t* keys, uint32_t* idx, uint32_t* flag) {
   const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (k >= nnz) return;
   const int32_t a = r[k], b = c[k];
@@ -370,9 +369,20 @@ __global__ __launch_bounds__(kT) void k_coo_keys(const int32_t* r, const int32_t
     keys[k] = 0;
   } else {
     keys[k] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
-    atomicAdd(&cnt[a], 1u);
   }
   idx[k] = (uint32_t)k;
+}
+
+// Row offsets of keys sorted by row: entry q starts every row in (row(q - 1), row(q)]; the last entry also
+// closes the rows after its own.
+__global__ __launch_bounds__(kT) void k_row_starts(const uint64_t* keys, int64_t nnz, int32_t rows, int32_t* rp) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz) return;
+  const int64_t r = (int64_t)(keys[q] >> 32);
+  const int64_t prev = q > 0 ? (int64_t)(keys[q - 1] >> 32) : -1;
+  for (int64_t x = prev + 1; x <= r; ++x) rp[x] = (int32_t)q;
+  if (q == nnz - 1)
+    for (int64_t x = r + 1; x <= rows; ++x) rp[x] = (int32_t)nnz;
 }
 
 template <class V>
@@ -405,15 +415,34 @@ __global__ __launch_bounds__(kT) void k_pair_keys(const uint64_t* keys, int64_t 
   pidx[pos[q]] = (uint32_t)q;
 }
 
-// odd[entry] = 1 for every entry of a pair stored an odd number of times (runs of the sorted pair keys)
-__global__ __launch_bounds__(kT) void k_pair_odd(const uint64_t* pk, const uint32_t* pidx, int64_t m, uint32_t* odd) {
+// odd[entry] = 1 for every entry of a pair stored an odd number of times (runs of the sorted pair keys).
+// An odd pair becomes symmetric (each entry gets its transpose); an even pair stays as stored, so the result
+// is structurally symmetric exactly when every even pair is stored as often in each direction: flag[0] = 1
+// where one is not.
+__global__ __launch_bounds__(kT) void k_pair_odd(const uint64_t* pk, const uint32_t* pidx, const uint64_t* keys,
+                                                int64_t m, uint32_t* odd, uint32_t* flag) {
   const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (s >= m) return;
   if (s > 0 && pk[s - 1] == pk[s]) return;  // not the start of a run
   int64_t e = s + 1;
-  while (e < m && pk[e] == pk[s]) ++e;
-  if (((e - s) & 1) == 0) return;
+  int64_t upper = 0;  // entries stored above the diagonal
+  for (;;) {
+    const uint64_t key = keys[pidx[e - 1]];
+    upper += (uint32_t)(key >> 32) < (uint32_t)key ? 1 : 0;
+    if (e >= m || pk[e] != pk[s]) break;
+    ++e;
+  }
+  if (((e - s) & 1) == 0) {
+    if (2 * upper != e - s) *flag = 1u;
+    return;
+  }
   for (int64_t x = s; x < e; ++x) odd[pidx[x]] = 1u;
+}
+
+// flag[0] = 1 for an empty row
+__global__ __launch_bounds__(kT) void k_empty_rows(const int32_t* rp, int32_t n, uint32_t* flag) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r < n && rp[r] == rp[r + 1]) *flag = 1u;
 }
 
 // COO in the insertion order of the symmetrisation: each entry, then its transpose when its pair is odd.
@@ -459,9 +488,13 @@ __global__ __launch_bounds__(kT) void k_struct_checks(const int32_t* rp, const i
   }
 }
 
-__global__ __launch_bounds__(kT) void k_cc_init(uint32_t* parent, int32_t n) {
+// parent[v] = the smallest neighbour of v if it is below v (sorted rows: the first column), else v. Each such
+// parent is an edge of the graph, so the forest starts partly joined.
+__global__ __launch_bounds__(kT) void k_cc_init(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
   const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (v < n) parent[v] = (uint32_t)v;
+  if (v >= n) return;
+  const int32_t a = rp[v];
+  parent[v] = (a < rp[v + 1] && ci[a] < (int32_t)v) ? (uint32_t)ci[a] : (uint32_t)v;
 }
 
 // Root of x with path halving. Parents only ever move to smaller indices, so every chain ends.
@@ -476,23 +509,39 @@ __device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t x) {
   return x;
 }
 
+// The same walk with loads and stores that may be served by this XCD's L2 (possibly stale). Every parent
+// value ever stored is below its node, so any value read is an ancestor of the node at some time, the walk
+// strictly decreases, and trees only merge: a stale walk ends at a node of the right tree.
+__device__ __forceinline__ uint32_t cc_find_cached(uint32_t* parent, uint32_t x) {
+  uint32_t p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (p != x) {
+    const uint32_t g = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (g != p) __hip_atomic_store(parent + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
 // Concurrent union-find over the edges (u, v), v < u: the larger root is hooked under the smaller one with a
-// compare-and-swap, so each tree's root is its smallest node.
+// device-coherent compare-and-swap, so each tree's root is its smallest node. Walks use cached loads; a CAS
+// only succeeds on a true root, and when it fails the walk restarts from the parent it returned, so the
+// larger of the two candidates strictly decreases and the loop ends.
 __global__ __launch_bounds__(kT) void k_cc_hook(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
   const int64_t u = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (u >= n) return;
   for (int32_t k = rp[u]; k < rp[u + 1]; ++k) {
     const uint32_t v = (uint32_t)ci[k];
     if (v >= (uint32_t)u) break;  // sorted columns: each undirected edge once
-    uint32_t a = cc_find(parent, (uint32_t)u), b = cc_find(parent, v);
+    uint32_t a = cc_find_cached(parent, (uint32_t)u), b = cc_find_cached(parent, v);
     while (a != b) {
       const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
       uint32_t expected = hi;
       if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT))
         break;
-      a = cc_find(parent, hi);
-      b = cc_find(parent, lo);
+      a = cc_find_cached(parent, expected);  // expected: hi's current parent, below hi
+      b = cc_find_cached(parent, lo);
     }
   }
 }
@@ -682,16 +731,18 @@ int coo_to_csr(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, 
   out.ci = x.template keep<int32_t>((size_t)nnz);
   out.vals = vals ? x.template keep<V>((size_t)nnz) : nullptr;
   if (x.failed()) return kOpsDevice;
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(out.rp);
-  x.run(k_fill_u32, groups((int64_t)rows + 1, kT), cnt, (int64_t)rows + 1, 0u);
   x.run(k_fill_u32, 1, flag, (int64_t)1, 0u);
-  if (nnz > 0) x.run(k_coo_keys, groups(nnz, kT), r, c, nnz, rows, cols, keys, idx, cnt, flag);
+  if (nnz > 0) x.run(k_coo_keys, groups(nnz, kT), r, c, nnz, rows, cols, keys, idx, flag);
   if (x.read_u32(flag)) return kOpsInvalid;
-  scan_u32(x, cnt, cnt, (int64_t)rows + 1, (uint32_t*)nullptr);
   radix_sort(x, keys, idx, keys2, idx2, nnz, bits_of(cols > 0 ? (uint64_t)(cols - 1) : 0),
              bits_of(rows > 0 ? (uint64_t)(rows - 1) : 0));
-  if (nnz > 0) x.run(k_csr_from_sorted<V>, groups(nnz, kT), (const uint64_t*)keys, (const uint32_t*)idx, nnz, vals,
-                     out.ci, out.vals);
+  if (nnz > 0) {
+    x.run(k_row_starts, groups(nnz, kT), (const uint64_t*)keys, nnz, rows, out.rp);
+    x.run(k_csr_from_sorted<V>, groups(nnz, kT), (const uint64_t*)keys, (const uint32_t*)idx, nnz, vals, out.ci,
+          out.vals);
+  } else {
+    x.run(k_fill_u32, groups((int64_t)rows + 1, kT), reinterpret_cast<uint32_t*>(out.rp), (int64_t)rows + 1, 0u);
+  }
   if (sorted_keys) *sorted_keys = keys;
   return x.failed() ? kOpsDevice : kOpsOk;
 }
@@ -706,6 +757,7 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
   uint64_t* skeys = nullptr;
   int rc = coo_to_csr(x, rows, cols, nnz, r, c, vals, a, &skeys);
   if (rc) return rc;
+  bool symmetric_known = false;  // the pair pass decided structural symmetry
   if (check_sym && nnz > 0) {
     uint32_t* pos = x.template alloc<uint32_t>((size_t)nnz + 1);
     uint32_t* tot = x.template alloc<uint32_t>(2);
@@ -717,6 +769,7 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
     uint32_t* odd = x.template alloc<uint32_t>((size_t)nnz + 1);
     if (x.failed()) return kOpsDevice;
     x.run(k_fill_u32, groups(nnz + 1, kT), odd, nnz + 1, 0u);
+    x.run(k_fill_u32, 1, tot, (int64_t)2, 0u);
     if (m > 0) {
       uint64_t* pk = x.template alloc<uint64_t>(m);
       uint64_t* pk2 = x.template alloc<uint64_t>(m);
@@ -726,8 +779,11 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
       x.run(k_pair_keys, groups(nnz, kT), (const uint64_t*)skeys, nnz, (const uint32_t*)pos, pk, pidx);
       const int nb = bits_of((uint64_t)(rows > cols ? rows : cols));
       radix_sort(x, pk, pidx, pk2, pidx2, (int64_t)m, nb, nb);
-      x.run(k_pair_odd, groups(m, kT), (const uint64_t*)pk, (const uint32_t*)pidx, (int64_t)m, odd);
+      x.run(k_pair_odd, groups(m, kT), (const uint64_t*)pk, (const uint32_t*)pidx, (const uint64_t*)skeys,
+            (int64_t)m, odd, tot);
     }
+    if (x.read_u32(tot)) return kOpsNotSymmetric;
+    symmetric_known = true;
     scan_u32(x, odd, odd, nnz + 1, tot + 1);
     const uint32_t extra = x.read_u32(tot + 1);
     if (x.failed()) return kOpsDevice;
@@ -754,10 +810,14 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
   uint32_t* root = x.template alloc<uint32_t>((size_t)n);
   if (x.failed()) return kOpsDevice;
   x.run(k_fill_u32, 1, flag, (int64_t)2, 0u);
-  x.run(k_struct_checks, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, flag);
-  if (x.read_u32(flag + 1)) return kOpsNotSymmetric;
+  if (symmetric_known) {
+    x.run(k_empty_rows, groups(n, kT), (const int32_t*)a.rp, n, flag);
+  } else {
+    x.run(k_struct_checks, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, flag);
+    if (x.read_u32(flag + 1)) return kOpsNotSymmetric;
+  }
   if (x.read_u32(flag)) return kOpsEmptyRow;
-  x.run(k_cc_init, groups(n, kT), root, n);
+  x.run(k_cc_init, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, root);
   x.run(k_cc_hook, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, root);
   x.run(k_cc_finish, groups(n, kT), root, n);
   uint32_t* tot = x.template alloc<uint32_t>(1);

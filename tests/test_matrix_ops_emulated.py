@@ -99,6 +99,9 @@ def test_canonicalize_refusals():
     c = np.array([1, 1, 0, 1, 2, 2], np.int32)
     rc, _ = E.canonicalize(3, 3, r, c, None, True)
     assert rc == -12
+    # without the symmetrisation the structure check finds it
+    rc, _ = E.canonicalize(3, 3, np.array([0, 0, 1, 2], np.int32), np.array([0, 1, 1, 2], np.int32), None, False)
+    assert rc == -12
     # the host repair follows the directed rows there and still succeeds
     _, links = P.prepare_rcm(P.convert_coo(3, 3, r, c, None), True)
     assert links >= 0
