@@ -270,6 +270,41 @@ def bench_pipeline(m, perm, steps: int, seed: int):
     return out
 
 
+def bench_variants_rmat(scale: int, steps: int, seed: int):
+    """The engine variants on a power-law graph: R-MAT (a, b, c) = (.57, .19, .19), edge factor 16, symmetrised,
+    repaired; kernel milliseconds of the batch engine's default (0) and widest (4) variants and of the
+    level-synchronous engine (5), and what the autotuner picks. Every variant gives the same order."""
+    import numpy as np
+
+    import parallelbatchrcm_amd as P
+
+    m = P.workloads.rmat(scale, seed=seed)
+    m, links = P.connect_components(m)
+    out = {"workload": f"R-MAT scale {scale} (n={m.rows}, nnz={m.nnz}, {links} repair links)"}
+    with P.RcmContext(timeout_ms=60000) as ctx:
+        ctx.load(m)
+        s = ctx.find_start()
+        ref = None
+        for v in (0, 4, 5):
+            ctx.set_param("variant", v)
+            ts = []
+            for _ in range(steps + 1):
+                ctx.run(s)
+                ts.append(ctx.timing().run_kernel_ms)
+            perm = ctx.perm()
+            ref = perm if ref is None else ref
+            out[f"variant{v}_ms"] = float(np.median(ts[1:]))
+            out[f"variant{v}_same_order"] = bool(np.array_equal(perm, ref))
+            out[f"variant{v}_steps"] = ctx.timing().batches  # batches (0-4) or BFS levels (5)
+        ctx.set_param("variant", 0)
+        tune = ctx.autotune(s, repeats=2)
+        out["autotune_variant"] = tune.variant
+        out["autotune_ms"] = tune.best_ms
+    print(f"[bench] R-MAT {scale}: variant 0 {out['variant0_ms']:.2f} ms, 4 {out['variant4_ms']:.2f} ms, "
+          f"level-synchronous {out['variant5_ms']:.2f} ms; autotune picks {tune.variant}", file=sys.stderr, flush=True)
+    return out
+
+
 COPY_BYTES = 256 << 20  # calibration copy for the HBM traffic counters
 
 
@@ -329,6 +364,7 @@ def main() -> int:
     ap.add_argument("--serving", type=int, default=400, help="serving form: requests per arrival rate (0 = skip)")
     ap.add_argument("--serving-rates", type=str, default="200,1000,4000")
     ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--rmat", type=int, default=20, help="R-MAT scale of the engine-variant leg (0 = skip)")
     ap.add_argument("--pipeline", type=int, default=5,
                     help="steps of the canonicalise and permute legs on the 128^3 stencil (0 = skip)")
     args = ap.parse_args()
@@ -422,6 +458,9 @@ def main() -> int:
 
     if args.pipeline > 0:
         line["pipeline"] = bench_pipeline(m, perm, args.pipeline, 9)
+
+    if args.rmat > 0:
+        line["variants_rmat"] = bench_variants_rmat(args.rmat, 3, args.rmat)
 
     if args.batched > 0:
         print(f"[bench] batched form: {args.batched} matrices", file=sys.stderr, flush=True)

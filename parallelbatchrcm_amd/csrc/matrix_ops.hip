@@ -1,5 +1,5 @@
-// matrix_ops.hip -- C ABI of include/rcm_matrix_ops.h: the executor that runs matrix_ops.hpp's kernels on a
-// HIP stream, with grow-only device arenas for scratch and results.
+// matrix_ops.hip -- C ABI of include/rcm_matrix_ops.h over matrix_ops.hpp's kernels, run by the HIP executor
+// of hip_exec.hpp on the handle's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,67 +9,17 @@
 #include <new>
 #include <vector>
 
+#include "hip_exec.hpp"
 #include "matrix_ops.hpp"
 #include "rcm_gpu_batch_abi.h"
 #include "rcm_matrix_ops.h"
-
-namespace {
-
-// Bump allocator over one device buffer. Requests beyond its capacity get their own hipMalloc for this call
-// and grow the buffer for the next one. Everything is released by reset().
-struct Arena {
-  char* base = nullptr;
-  size_t cap = 0, used = 0, wanted = 0;
-  std::vector<void*> extra;
-  bool failed = false;
-
-  void* get(size_t bytes) {
-    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
-    wanted += bytes;
-    if (base && used + bytes <= cap) {
-      void* p = base + used;
-      used += bytes;
-      return p;
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      failed = true;
-      return nullptr;
-    }
-    extra.push_back(p);
-    return p;
-  }
-  // Frees this call's overflow and grows the buffer to what the call needed.
-  void reset() {
-    for (void* p : extra) (void)hipFree(p);
-    extra.clear();
-    if (wanted > cap) {
-      if (base) (void)hipFree(base);
-      base = nullptr;
-      cap = 0;
-      const size_t want = wanted + wanted / 4;
-      if (hipMalloc(reinterpret_cast<void**>(&base), want) == hipSuccess) cap = want;
-    }
-    used = wanted = 0;
-    failed = false;
-  }
-  void release() {
-    for (void* p : extra) (void)hipFree(p);
-    extra.clear();
-    if (base) (void)hipFree(base);
-    base = nullptr;
-    cap = used = wanted = 0;
-  }
-};
-
-}  // namespace
 
 struct rcm_ops {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {};
-  Arena scratch;  // per call
-  Arena result;   // the last canonicalisation's CSR (and its intermediates)
+  rcm_mx::DevArena scratch;  // per call
+  rcm_mx::DevArena result;   // the last canonicalisation's CSR (and its intermediates)
   uint32_t* h_word = nullptr;  // pinned readback word
   double last_ms = 0.0;
   // last canonicalisation
@@ -83,39 +33,7 @@ struct rcm_ops {
 
 namespace {
 
-struct HipExec {
-  rcm_ops* o;
-  bool bad = false;
-
-  template <class K, class... A>
-  void run(K kernel, int64_t grid, A... args) {
-    if (bad || grid <= 0) return;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(rcm_mx::kT), 0, o->stream, args...);
-    if (hipGetLastError() != hipSuccess) bad = true;
-  }
-  template <class T>
-  T* alloc(size_t count) {
-    T* p = static_cast<T*>(o->scratch.get(count * sizeof(T)));
-    if (!p) bad = true;
-    return p;
-  }
-  template <class T>
-  T* keep(size_t count) {
-    T* p = static_cast<T*>(o->result.get(count * sizeof(T)));
-    if (!p) bad = true;
-    return p;
-  }
-  uint32_t read_u32(const uint32_t* p) {
-    if (bad) return 0;
-    if (hipMemcpyAsync(o->h_word, p, sizeof(uint32_t), hipMemcpyDeviceToHost, o->stream) != hipSuccess ||
-        hipStreamSynchronize(o->stream) != hipSuccess) {
-      bad = true;
-      return 0;
-    }
-    return *o->h_word;
-  }
-  bool failed() const { return bad; }
-};
+using rcm_mx::HipExec;
 
 int to_status(int ops_rc) {
   switch (ops_rc) {
@@ -131,7 +49,7 @@ int to_status(int ops_rc) {
 template <class F>
 int timed(rcm_ops* o, F&& body) {
   if (hipSetDevice(o->device) != hipSuccess) return RCM_EDEVICE;
-  HipExec x{o};
+  HipExec x{o->stream, &o->scratch, &o->result, o->h_word};
   if (hipEventRecord(o->ev[0], o->stream) != hipSuccess) return RCM_EDEVICE;
   int rc = body(x);
   if (hipEventRecord(o->ev[1], o->stream) != hipSuccess || hipStreamSynchronize(o->stream) != hipSuccess)
@@ -159,7 +77,8 @@ int rcm_ops_create(int device, rcm_ops** out) {
   o->device = device;
   if (hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&o->ev[0]) != hipSuccess || hipEventCreate(&o->ev[1]) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&o->h_word), sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&o->h_word), rcm_mx::kPinnedWords * sizeof(uint32_t),
+                    hipHostMallocDefault) != hipSuccess) {
     rcm_ops_destroy(o);
     return RCM_EDEVICE;
   }

@@ -16,6 +16,7 @@
 #include "device_common.hpp"
 
 namespace rcm_mx {
+namespace {  // internal linkage: each library source that includes this has its own copy
 
 using rcm::ballot;
 using rcm::kWave;
@@ -847,4 +848,5 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
   return x.failed() ? kOpsDevice : kOpsOk;
 }
 
+}  // namespace
 }  // namespace rcm_mx

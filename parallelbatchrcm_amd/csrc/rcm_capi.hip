@@ -19,6 +19,8 @@
 #include <vector>
 
 #include "engine_many.hpp"
+#include "hip_exec.hpp"
+#include "level_cm.hpp"
 #include "rcm_gpu_batch_abi.h"
 
 using namespace rcm;
@@ -40,6 +42,11 @@ template <> struct Var<4> { using G = Geo<512, 4096, 512>; static constexpr int 
 constexpr int kVarThreads[kVariants] = {256, 256, 128, 256, 512};
 constexpr int kVarSlots[kVariants] = {2048, 4096, 1024, 2048, 4096};
 constexpr int kVarDigit[kVariants] = {4, 4, 4, 6, 4};
+// Variant kLevelVariant of a single-matrix context is the level-synchronous engine (level_cm.hpp): one BFS
+// level per step, ordered by a device-wide radix sort with 8-bit digits.
+constexpr int kLevelVariant = kVariants;
+constexpr int kLevelThreads = 256;
+constexpr int kLevelDigit = 8;
 using GeoSmall = Var<0>::G;  // the many-matrix helpers' default
 constexpr int kDigit = 4;
 
@@ -132,6 +139,8 @@ struct rcm_ctx {
   Control* h_ctl = nullptr;     // pinned
   int32_t dirty = 0;  // batch records [0, dirty) may hold values from earlier runs
   int max_resident[kVariants] = {};  // co-resident workgroups of each variant
+  rcm_mx::DevArena lscratch;         // level-synchronous engine scratch
+  uint32_t* h_word = nullptr;        // pinned readback word of that engine
   // last result
   bool have_perm = false;
   bool reversed = false;
@@ -175,15 +184,43 @@ int wait_stream(rcm_ctx* c, hipEvent_t ev, int64_t limit_ms) {
 }
 
 int workers_for(rcm_ctx* c) {
+  if (c->variant == kLevelVariant) return 0;
   int w = (int)c->workers;
   if (w <= 0) w = c->n > 16384 ? 256 : 64;  // measured on MI355X: 128-256 is best from 65k to 2M nodes
   return std::max(1, std::min(w, c->max_resident[c->variant]));
+}
+
+int variant_threads(int64_t v) { return v == kLevelVariant ? kLevelThreads : kVarThreads[v]; }
+
+// The level-synchronous engine: same outputs as run_engine (perm, qinfo, and levels for the plain BFS).
+int run_level(rcm_ctx* c, int32_t start, bool bfs) {
+  rcm_mx::HipExec x{c->stream, &c->lscratch, &c->lscratch, c->h_word};
+  RCM_HIP(hipEventRecord(c->ev[0], c->stream));
+  int32_t placed = 0, levels = 0;
+  const int st = rcm_mx::level_order(x, c->n, c->nnz, c->rp, c->ci, c->maxdeg, start, c->perm, c->qinfo, c->claim,
+                                     bfs ? c->level : nullptr, &placed, &levels);
+  RCM_HIP(hipEventRecord(c->ev[2], c->stream));
+  const bool synced = hipStreamSynchronize(c->stream) == hipSuccess;
+  c->lscratch.reset();
+  if (!synced || st == rcm_mx::kOpsDevice) return RCM_EDEVICE;
+  if (st == rcm_mx::kOpsCapacity) return RCM_ECAPACITY;
+  float ms = 0.f;
+  RCM_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[2]));
+  c->timing.init_kernel_ms = 0.0;
+  c->timing.run_kernel_ms = ms;
+  c->timing.workers = 0;
+  c->timing.batches = levels;
+  c->timing.placed = placed;
+  c->timing.error_site = 0;
+  if (placed != c->n) return RCM_ENOTCONNECTED;
+  return RCM_OK;
 }
 
 // One engine run from `start` (CM order, or plain BFS order with levels when bfs).
 int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   if (!c->rp) return RCM_ENOTLOADED;
   if (start < 0 || start >= c->n) return RCM_EINVAL;
+  if (c->variant == kLevelVariant) return run_level(c, start, bfs);
   RunArgs a{};
   a.row_ptr = c->rp;
   a.col_idx = c->ci;
@@ -705,6 +742,8 @@ int rcm_ctx_create(int device, rcm_ctx** out) {
   ok = ok && dev_alloc(&c->ctl, 1) == hipSuccess && dev_alloc(&c->scratch, 16) == hipSuccess &&
        dev_alloc(&c->prof, kProfSlots) == hipSuccess;
   ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(Control), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->h_word), rcm_mx::kPinnedWords * sizeof(uint32_t),
+                            hipHostMallocDefault) == hipSuccess;
   hipDeviceProp_t prop;
   ok = ok && hipGetDeviceProperties(&prop, device) == hipSuccess;
   for (int v = 0; ok && v < kVariants; ++v) {
@@ -742,6 +781,8 @@ int rcm_ctx_destroy(rcm_ctx* c) {
   dev_free(c->scratch);
   dev_free(c->prof);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+  if (c->h_word) (void)hipHostFree(c->h_word);
+  c->lscratch.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -756,8 +797,9 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_TIMEOUT_MS: if (v <= 0) return RCM_EINVAL; c->timeout_ms = v; return RCM_OK;
     case RCM_PARAM_SORT: if (v < 0 || v > 2) return RCM_EINVAL; c->sort_mode = v; return RCM_OK;
     case RCM_PARAM_CLAIM_FILTER: if (v < 0 || v > 2) return RCM_EINVAL; c->claim_filter = v; return RCM_OK;
-    case RCM_PARAM_DIGIT_BITS: return v == kVarDigit[c->variant] ? RCM_OK : RCM_EINVAL;  // part of the variant
-    case RCM_PARAM_VARIANT: if (v < 0 || v >= kVariants) return RCM_EINVAL; c->variant = v; return RCM_OK;
+    case RCM_PARAM_DIGIT_BITS:  // part of the variant
+      return v == (c->variant == kLevelVariant ? kLevelDigit : kVarDigit[c->variant]) ? RCM_OK : RCM_EINVAL;
+    case RCM_PARAM_VARIANT: if (v < 0 || v > kLevelVariant) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
     case RCM_PARAM_PROFILE: c->profile = v ? 1 : 0; return RCM_OK;
     default: return RCM_EINVAL;
@@ -779,7 +821,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_TIMEOUT_MS: *v = c->timeout_ms; return RCM_OK;
     case RCM_PARAM_SORT: *v = c->sort_mode; return RCM_OK;
     case RCM_PARAM_CLAIM_FILTER: *v = c->claim_filter; return RCM_OK;
-    case RCM_PARAM_DIGIT_BITS: *v = kVarDigit[c->variant]; return RCM_OK;
+    case RCM_PARAM_DIGIT_BITS: *v = c->variant == kLevelVariant ? kLevelDigit : kVarDigit[c->variant]; return RCM_OK;
     case RCM_PARAM_VARIANT: *v = c->variant; return RCM_OK;
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
     case RCM_PARAM_PROFILE: *v = c->profile; return RCM_OK;
@@ -1586,10 +1628,32 @@ int rcm_ctx_autotune(rcm_ctx* c, int32_t start, int32_t repeats, rcm_tune_result
       }
     }
   }
+  {  // the level-synchronous engine: one configuration
+    c->variant = kLevelVariant;
+    std::vector<double> t;
+    int rc = RCM_OK;
+    for (int r = 0; r < repeats && rc == RCM_OK; ++r) {
+      rc = run_engine(c, start, false);
+      t.push_back(c->timing.run_kernel_ms);
+    }
+    if (rc == RCM_OK) {  // a level too wide for 64-bit keys leaves it out
+      ++count;
+      const double ms = median_of(t);
+      if (ms < best.best_ms) {
+        best.best_ms = ms;
+        best.variant = kLevelVariant;
+        best.workers = 0;
+      }
+    } else if (rc != RCM_ECAPACITY) {
+      c->variant = keep_v;
+      c->workers = keep_w;
+      return rc;
+    }
+  }
   c->variant = best.variant;
   c->workers = best.workers;
   best.candidates = count;
-  best.threads = kVarThreads[best.variant];
+  best.threads = variant_threads(best.variant);
   c->have_perm = false;  // the tuning runs overwrote the last order
   *out = best;
   return RCM_OK;

@@ -3,6 +3,7 @@
 // Built by tests/emu/Makefile into tests/emu/libemu_ops.so.
 #include "wave_emu.hpp"
 #include "matrix_ops.hpp"
+#include "level_cm.hpp"
 
 #include <cstdlib>
 #include <cstring>
@@ -38,6 +39,9 @@ struct EmuExec {
   template <class T>
   T* keep(size_t count) { return get<T>(kept, count); }
   uint32_t read_u32(const uint32_t* p) { return bad ? 0u : *p; }
+  void read_words(uint32_t* dst, const uint32_t* src, int count) {
+    if (!bad) std::memcpy(dst, src, (size_t)count * sizeof(uint32_t));
+  }
   bool failed() const { return bad; }
   ~EmuExec() {
     for (void* p : scratch) std::free(p);
@@ -90,6 +94,18 @@ int emu_radix_sort(int64_t n, uint64_t* keys, uint32_t* vals, int32_t lo_bits, i
   std::memcpy(keys, k, (size_t)n * 8);
   std::memcpy(vals, v, (size_t)n * 4);
   return 0;
+}
+
+// Level-synchronous order from start (bfs: plain BFS with levels). Returns the status; *placed, *levels.
+int emu_level_order(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_t start, int32_t bfs,
+                    int32_t* perm, int32_t* level, int32_t* placed, int32_t* levels) {
+  EmuExec x;
+  uint32_t maxdeg = 0;
+  for (int32_t i = 0; i < n; ++i) maxdeg = std::max<uint32_t>(maxdeg, (uint32_t)(rp[i + 1] - rp[i]));
+  uint64_t* qinfo = x.alloc<uint64_t>((size_t)n);
+  uint64_t* claim = x.alloc<uint64_t>((size_t)n);
+  return rcm_mx::level_order(x, n, nnz, rp, ci, maxdeg, start, perm, qinfo, claim, bfs ? level : nullptr, placed,
+                             levels);
 }
 
 // Exclusive scan of u32 with the total appended (out: m + 1 entries).

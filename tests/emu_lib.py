@@ -100,6 +100,8 @@ def _ops_lib():
                                        I64P, I64P, I32P, I32P, DP]
         L.emu_radix_sort.restype = C.c_int
         L.emu_radix_sort.argtypes = [C.c_int64, U64P, U32P, C.c_int32, C.c_int32]
+        L.emu_level_order.restype = C.c_int
+        L.emu_level_order.argtypes = [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_int32, I32P, I32P, I32P, I32P]
         L.emu_scan.restype = C.c_int
         L.emu_scan.argtypes = [C.c_int64, U32P, U32P]
         L._ops_ready = True
@@ -155,3 +157,18 @@ def canonicalize(rows, cols, r, c, vals, check_sym):
         return rc, None
     m = nnz.value
     return 0, (n.value, rp[: n.value + 1].copy(), ci[:m].copy(), None if v is None else v[:m].copy(), links.value)
+
+
+def level_order(rp, ci, start, bfs=False):
+    """Level-synchronous engine (level_cm.hpp): (status, perm[:placed], level[:placed] or None, levels)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    n = rp.size - 1
+    perm = np.full(n, -1, np.int32)
+    lev = np.full(n, -1, np.int32)
+    placed, levels = C.c_int32(), C.c_int32()
+    rc = _ops_lib().emu_level_order(n, ci.size, rp.ctypes.data_as(I32P), ci.ctypes.data_as(I32P), int(start),
+                                    1 if bfs else 0, perm.ctypes.data_as(I32P), lev.ctypes.data_as(I32P),
+                                    C.byref(placed), C.byref(levels))
+    k = placed.value
+    return rc, perm[:k], (lev[:k] if bfs else None), levels.value

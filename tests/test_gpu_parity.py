@@ -228,7 +228,7 @@ def test_claim_filter_modes_agree(ctx, flt):
         ctx.set_param("claim_filter", 0)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_engine_variants_match_oracle(ctx, variant):
     """Every engine variant (block size, batch width, radix digit) gives the same order."""
     cases = [P.CSR(*G.random_connected(500, 1500, seed=4)), P.workloads.config(2, scale=24, seed=2),
@@ -253,10 +253,43 @@ def test_autotune_keeps_results(ctx):
     ctx.load(m)
     try:
         r = ctx.autotune(s, repeats=2)
-        assert 0 <= r.variant < 5 and r.workers >= 1 and r.candidates >= 10
+        assert 0 <= r.variant <= 5 and r.candidates >= 11
+        assert r.workers >= 1 if r.variant < 5 else r.workers == 0
         assert 0 < r.best_ms <= r.default_ms
         ctx.run(s)
         assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
     finally:
         ctx.set_param("variant", 0)
         ctx.set_param("workers", 0)
+
+
+def test_level_synchronous_variant(ctx):
+    """Variant 5, the level-synchronous engine: small graphs from several starts (duplicates, unsorted rows),
+    hub graphs, R-MAT, the start search, reverse and a disconnected graph."""
+    try:
+        ctx.set_param("variant", 5)
+        for name, (rp, ci) in SMALL.items():
+            n = rp.shape[0] - 1
+            ctx.load(_csr(rp, ci))
+            for st in sorted({0, n // 2, n - 1}):
+                _say("level variant", name, st)
+                ctx.run(st)
+                np.testing.assert_array_equal(ctx.perm(), O.cm(rp, ci, st), err_msg=f"{name} {st}")
+        for name in ("hub5000", "hubdups", "rmat14", "rmat16"):
+            m = LONG[name]()
+            rc, ps, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+            _say("level variant", name)
+            ctx.load(m)
+            assert ctx.find_start() == ps
+            ctx.run(ps, P.RCM_FLAG_REVERSE)
+            want = O.cm(m.row_offsets, m.col_ids, ps)
+            np.testing.assert_array_equal(ctx.perm(), want[::-1])
+            assert ctx.bandwidth() == O.bandwidth(m.row_offsets, m.col_ids, want)
+        rp, ci = G.path(10)
+        rp2 = np.concatenate([rp, rp[1:] + rp[-1]])
+        ci2 = np.concatenate([ci, ci + 10])
+        _say("level variant disconnected")
+        ctx.load(_csr(rp2, ci2))
+        assert ctx.run_status(0) == P.RCM_ENOTCONNECTED
+    finally:
+        ctx.set_param("variant", 0)

@@ -8,7 +8,9 @@ import numpy as np
 import pytest
 
 import emu_lib as E
+import graphs as G
 import matrix_ref as M
+import oracle_lib as O
 import parallelbatchrcm_amd as P
 
 
@@ -114,3 +116,31 @@ def test_canonicalize_refusals():
     # non-square with nothing to mirror
     rc, _ = E.canonicalize(2, 3, np.array([0, 1], np.int32), np.array([0, 1], np.int32), None, False)
     assert rc == -1
+
+
+LEVEL_CASES = {
+    "path17": G.path(17),
+    "star40": G.star(40),
+    "grid7x5": G.grid2d(7, 5),
+    "rand500": G.random_connected(500, 1500, seed=4),
+    "dups300": G.with_duplicates(*G.random_connected(300, 600, seed=5), seed=6),
+    "unsorted300": G.shuffle_rows(*G.random_connected(300, 900, seed=7), seed=8),
+    "hub900": G.hub_graph(900, 1500, seed=2),  # a level of 900 nodes: the wide path
+    "star9000": G.star(9000),  # one parent with 9000 entries: beyond a narrow level's work
+}
+
+
+@pytest.mark.parametrize("name", list(LEVEL_CASES))
+def test_level_synchronous_engine_matches_oracle(name):
+    """level_cm.hpp: the CM order and the plain BFS order with levels, from several starts."""
+    rp, ci = LEVEL_CASES[name]
+    n = rp.shape[0] - 1
+    for s in sorted({0, n // 2, n - 1}):
+        rc, perm, _, _ = E.level_order(rp, ci, s)
+        assert rc == 0
+        np.testing.assert_array_equal(perm, O.cm(rp, ci, s))
+        rc, order, lev, depth = E.level_order(rp, ci, s, bfs=True)
+        want, wlev = O.bfs(rp, ci, s)
+        np.testing.assert_array_equal(order, want)
+        np.testing.assert_array_equal(lev, wlev)
+        assert depth == wlev.max() + 1
