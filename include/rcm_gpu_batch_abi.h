@@ -74,7 +74,11 @@ enum rcm_param {
                                 1 = 256 threads, 4096 slots, 256 parents, 4-bit digits
                                 2 = 128 threads, 1024 slots, 128 parents, 4-bit digits
                                 3 = 256 threads, 2048 slots, 256 parents, 6-bit digits
-                                4 = 512 threads, 4096 slots, 512 parents, 4-bit digits */
+                                4 = 512 threads, 4096 slots, 512 parents, 4-bit digits
+                                5 = level-synchronous engine (rcm_ctx only): one BFS level per step, children
+                                    ordered by a device-wide radix sort with 8-bit digits; for wide, shallow
+                                    graphs. The structure of GPU_SIMPLE (src/device/rcm_gpu.cu:134-330) with
+                                    the serial order. workers is 0. */
   RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
   RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
   RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
@@ -240,7 +244,8 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
                        int64_t* bandwidth_out, int32_t* status_out);
 
 /* ---- autotuner ---- */
-/* No reference counterpart. Times every candidate (engine variant x persistent workgroups) on the loaded work
+/* No reference counterpart. Times every candidate (engine variant x persistent workgroups, and for rcm_ctx the
+ * level-synchronous variant 5) on the loaded work
  * (`repeats` runs each, median of the device time) and keeps the fastest in the context; the order never
  * depends on the choice. rcm_ctx_autotune runs the matrix from `start`; rcm_batch_autotune runs the whole
  * loaded set from start[k]. */
