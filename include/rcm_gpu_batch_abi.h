@@ -243,6 +243,12 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
                        const int32_t* start, uint32_t flags, const int64_t* po_off, int32_t* perm_cat,
                        int64_t* bandwidth_out, int32_t* status_out);
 
+/* ---- diagnostics ---- */
+/* No reference counterpart. Runs the wave64 and workgroup primitives the engines are built on once, on one
+ * 256-thread workgroup: in holds 256 words; out receives 6 x 256 words: the block exclusive scan, the block
+ * total (in every word), and per 64-lane wave the sum, the maximum, the minimum and the inclusive scan. */
+int rcm_selftest_primitives(int device, const uint32_t* in, uint32_t* out);
+
 /* ---- autotuner ---- */
 /* No reference counterpart. Times every candidate (engine variant x persistent workgroups, and for rcm_ctx the
  * level-synchronous variant 5) on the loaded work

@@ -158,6 +158,7 @@ _SIGNATURES = {
         [C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P, C.c_uint32, I64P, I32P, I64P, I32P],
     ),
     "rcm_status_string": (C.c_char_p, [C.c_int]),
+    "rcm_selftest_primitives": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     # rcm_workloads.h
     "rcm_csr_free": (None, [C.POINTER(CsrT)]),
     "rcm_gen_stencil2d": (C.c_int32, [C.c_int32, C.c_int32, C.POINTER(CsrT)]),

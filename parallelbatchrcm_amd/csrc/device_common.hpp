@@ -72,28 +72,44 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 // (one wave-wide load of one address, s_memrealtime) so that the control flow they steer is visibly uniform.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// DPP row moves (gfx9 family): row_shr:n = 0x110 + n within rows of 16 lanes, row_bcast:15 = 0x142 (lane 15 of
+// each row to the next row), row_bcast:31 = 0x143 (lane 31 to rows 2 and 3). Lanes without a source, or
+// outside ROW_MASK, receive `identity`. One VALU instruction each, against an LDS round trip for a shuffle.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v, uint32_t identity) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane63(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1); }
+
+// Inclusive prefix sum over the wave: Hillis-Steele within each row of 16, then the row totals.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t t = __shfl_up(v, o, kWave);
-    if (lane_id() >= o) v += t;
-  }
+  v += dpp_move<0x111>(v, 0u);
+  v += dpp_move<0x112>(v, 0u);
+  v += dpp_move<0x114>(v, 0u);
+  v += dpp_move<0x118>(v, 0u);
+  v += dpp_move<0x142, 0xa>(v, 0u);
+  v += dpp_move<0x143, 0xc>(v, 0u);
   return v;
 }
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
-}
+// Reductions: the same prefix pattern with max / min; lane 63 holds the result, returned to every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return lane63(wave_incl_scan(v)); }
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
-  return v;
+  v = max(v, dpp_move<0x111>(v, 0u));
+  v = max(v, dpp_move<0x112>(v, 0u));
+  v = max(v, dpp_move<0x114>(v, 0u));
+  v = max(v, dpp_move<0x118>(v, 0u));
+  v = max(v, dpp_move<0x142, 0xa>(v, 0u));
+  v = max(v, dpp_move<0x143, 0xc>(v, 0u));
+  return lane63(v);
 }
 __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, kWave));
-  return v;
+  v = min(v, dpp_move<0x111>(v, ~0u));
+  v = min(v, dpp_move<0x112>(v, ~0u));
+  v = min(v, dpp_move<0x114>(v, ~0u));
+  v = min(v, dpp_move<0x118>(v, ~0u));
+  v = min(v, dpp_move<0x142, 0xa>(v, ~0u));
+  v = min(v, dpp_move<0x143, 0xc>(v, ~0u));
+  return lane63(v);
 }
 
 // Block-wide exclusive scan of one value per thread; `wtmp` needs W+1 words of LDS.

@@ -86,6 +86,25 @@ void* many_kernel(int v, bool bfs) {
   }
 }
 
+// Self-test of the wave64 and block primitives the engines are built on (rcm_selftest_primitives): one
+// 256-thread workgroup; out = [block exclusive scan | block total | wave sum | wave max | wave min |
+// wave inclusive scan], 256 words each (the total at word 256).
+constexpr int kSelftestThreads = 256;
+constexpr int kSelftestIn = kSelftestThreads;
+constexpr int kSelftestOut = 6 * kSelftestThreads;
+__global__ __launch_bounds__(kSelftestThreads) void k_selftest(const uint32_t* in, uint32_t* out) {
+  __shared__ uint32_t wtmp[kSelftestThreads / kWave + 1];
+  const int t = threadIdx.x;
+  const uint32_t v = in[t];
+  uint32_t total = 0;
+  out[t] = block_excl_scan<kSelftestThreads>(v, wtmp, &total);
+  out[kSelftestThreads + t] = total;
+  out[2 * kSelftestThreads + t] = wave_sum(v);
+  out[3 * kSelftestThreads + t] = wave_max(v);
+  out[4 * kSelftestThreads + t] = wave_min(v);
+  out[5 * kSelftestThreads + t] = wave_incl_scan(v);
+}
+
 #define RCM_HIP(call)                         \
   do {                                        \
     if ((call) != hipSuccess) return RCM_EDEVICE; \
@@ -708,6 +727,25 @@ void lane_loop(rcm_server* s, int lane) {
 }  // namespace
 
 extern "C" {
+
+int rcm_selftest_primitives(int device, const uint32_t* in, uint32_t* out) {
+  if (!in || !out) return RCM_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return RCM_EDEVICE;
+  RCM_HIP(hipSetDevice(device));
+  uint32_t* d = nullptr;
+  if (dev_alloc(&d, kSelftestIn + kSelftestOut) != hipSuccess) return RCM_ENOMEM;
+  int rc = RCM_OK;
+  if (hipMemcpy(d, in, kSelftestIn * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) rc = RCM_EDEVICE;
+  if (rc == RCM_OK) {
+    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(kSelftestThreads), 0, nullptr, (const uint32_t*)d, d + kSelftestIn);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, d + kSelftestIn, kSelftestOut * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = RCM_EDEVICE;
+  }
+  dev_free(d);
+  return rc;
+}
 
 const char* rcm_status_string(int s) {
   switch (s) {
