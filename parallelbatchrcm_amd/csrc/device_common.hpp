@@ -132,6 +132,32 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtmp, 
   return base + inc - v;
 }
 
+// Exclusive scan of v and maximum of mx over the block in one pass; `wtmp` needs 2W words. Two barriers.
+template <int T>
+__device__ __forceinline__ uint32_t block_excl_scan_max(uint32_t v, uint32_t mx, uint32_t* wtmp, uint32_t* total,
+                                                        uint32_t* max_out) {
+  constexpr int W = T / kWave;
+  const uint32_t inc = wave_incl_scan(v);
+  const uint32_t wm = wave_max(mx);
+  if (lane_id() == kWave - 1) {
+    wtmp[wave_id()] = inc;
+    wtmp[W + wave_id()] = wm;
+  }
+  __syncthreads();
+  uint32_t base = 0, sum = 0, m = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint32_t sw = wtmp[w];
+    base += (w < wave_id()) ? sw : 0u;
+    sum += sw;
+    m = max(m, wtmp[W + w]);
+  }
+  __syncthreads();
+  *total = sum;
+  *max_out = m;
+  return base + inc - v;
+}
+
 // Block-wide reduction (max) of one value per thread; `wtmp` needs W words. Two barriers.
 template <int T>
 __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* wtmp) {

@@ -671,9 +671,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     const bool ok = claims_complete_before(a, (int64_t)b);
     if (lane_id() == 0) sm.early = ok ? 1 : 0;
   }
-  uint32_t D;
-  const uint32_t soff = block_excl_scan<T>(d, sm.wtmp, &D);
-  const uint32_t maxpdeg = block_max<T>(d, sm.wtmp);
+  uint32_t D, maxpdeg;
+  const uint32_t soff = block_excl_scan_max<T>(d, d, sm.wtmp, &D, &maxpdeg);
   if (tid < m) sm.so[tid] = soff;
   if (tid == 0) sm.so[m] = D;
   __syncthreads();
@@ -805,9 +804,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       lmax = max(lmax, sm.sre[j] - sm.srb[j]);
     }
   }
-  uint32_t cnt;
-  uint32_t pos = block_excl_scan<T>(lc, sm.wtmp, &cnt);
-  const uint32_t maxdeg = block_max<T>(lmax, sm.wtmp);
+  uint32_t cnt, maxdeg;
+  uint32_t pos = block_excl_scan_max<T>(lc, lmax, sm.wtmp, &cnt, &maxdeg);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const uint32_t j = (uint32_t)(tid * EPT + k);
