@@ -83,7 +83,11 @@ enum rcm_param {
   RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
   RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
   RCM_PARAM_CLAIM_FILTER = 9 /* 0 = atomicMin on every adjacency entry; 1 = read the claim word first and skip
-                                entries already claimed by a smaller code (plain load); 2 = same, device-scope */
+                                entries already claimed by a smaller code (plain load); 2 = same, device-scope */,
+  RCM_PARAM_WG_MAX_N = 10  /* rcm_batch only: matrices with at most this many rows and rows of at most 64 entries
+                              are ordered by one workgroup each, level by level (no cross-workgroup chain);
+                              the others by the batch engine. 0 = never (the default: that engine is not
+                              cleared for GPU use). */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);

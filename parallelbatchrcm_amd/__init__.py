@@ -67,6 +67,7 @@ PARAM_STRICT = 6
 PARAM_PROFILE = 7
 PARAM_GRAPH = 8
 PARAM_CLAIM_FILTER = 9
+PARAM_WG_MAX_N = 10
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -658,7 +659,7 @@ class RcmBatch:
         self._bs: Optional[BatchSet] = None
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH,
-               "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT}
+               "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT, "wg_max_n": PARAM_WG_MAX_N}
 
     def autotune(self, starts, repeats: int = 1) -> TuneResultT:
         """Time every engine variant x workgroup count on the loaded set and keep the fastest."""

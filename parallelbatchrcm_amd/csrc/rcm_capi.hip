@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "engine_many.hpp"
+#include "engine_wg.hpp"
 #include "hip_exec.hpp"
 #include "level_cm.hpp"
 #include "rcm_gpu_batch_abi.h"
@@ -29,6 +30,11 @@ namespace {
 
 constexpr int kDefaultSegThreshold = 24;
 constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56% fewer HBM writes
+// Many-matrix launches: matrices with at most this many rows, and rows of at most kWgMaxRow entries, are
+// ordered by one workgroup each (engine_wg.hpp); the others by the batch engine's helpers. Off (0): the first
+// GPU run of that engine ended in a memory fault whose cause has not been shown; the engine's device code is
+// exercised only by the CPU emulator tests.
+constexpr int64_t kDefaultWgMaxN = 0;
 
 // Engine variants (RCM_PARAM_VARIANT): block size T, batch width (adjacency slots S, parents P), radix digit.
 // The order never depends on the variant (batch boundaries do not change it); the speed does.
@@ -329,6 +335,8 @@ struct rcm_batch {
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
+  int64_t wg_max_n = kDefaultWgMaxN;
+  int max_res_wg = 0;                // co-resident workgroups of the one-workgroup-per-matrix engine
   int64_t variant = 0;
   int max_resident = 0;              // of the current variant
   int max_res[kVariants] = {};       // co-resident workgroups of each variant
@@ -365,6 +373,10 @@ struct rcm_batch {
   bool have_perm = false;
   bool reversed = false;
   std::vector<int32_t> ok_jobs;  // matrices whose last run completed
+  std::vector<uint32_t> maxrow;  // longest row per matrix (from the load's validation)
+  std::vector<uint8_t> wg;       // per matrix: ordered by one workgroup
+  int32_t n_helper = 0;          // `order` lists the helper engine's matrices first, then the others
+  std::vector<int32_t> blk_order_h;  // host copy of the block map of `order`
   rcm_batch_timing_t timing{};
 };
 
@@ -494,13 +506,56 @@ int batch_workers(rcm_batch* b) {
   return std::max(1, std::min(w, b->max_resident));
 }
 
-void enqueue_many(rcm_batch* b, const int32_t* d_order, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs) {
-  const ManyArgs m = batch_args(b, d_order, count);
-  hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, m, d_blk);
-  void* kern = many_kernel((int)b->variant, bfs);
-  ManyArgs mm = m;
-  void* args[] = {&mm};
-  (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
+// One pass over a list of matrices: d_list[0, nhelper) by the batch engine's helpers (d_blk's first nhelper + 1
+// entries are their block map, nblk_helper blocks), then d_list[nhelper, nhelper + nwg) with one workgroup each.
+void enqueue_many(rcm_batch* b, const int32_t* d_list, const int32_t* d_blk, int32_t nblk_helper, int32_t nhelper,
+                  int32_t nwg, bool bfs) {
+  if (nhelper > 0) {
+    const ManyArgs m = batch_args(b, d_list, nhelper);
+    hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk_helper), dim3(256), 0, b->stream, m, d_blk);
+    void* kern = many_kernel((int)b->variant, bfs);
+    ManyArgs mm = m;
+    void* args[] = {&mm};
+    (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
+  }
+  if (nwg > 0) {
+    const ManyArgs m = batch_args(b, d_list + nhelper, nwg);
+    hipLaunchKernelGGL(k_wg_prepare, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0, b->stream, m);
+    const unsigned grid = (unsigned)std::min(nwg, b->max_res_wg);
+    if (bfs)
+      hipLaunchKernelGGL(k_order_wg<true>, dim3(grid), dim3(kWgT), 0, b->stream, m);
+    else
+      hipLaunchKernelGGL(k_order_wg<false>, dim3(grid), dim3(kWgT), 0, b->stream, m);
+  }
+}
+
+bool wg_eligible(const rcm_batch* b, int32_t k) {
+  return b->wg_max_n > 0 && (int64_t)b->n[k] <= b->wg_max_n && b->maxrow[k] <= (uint32_t)kWgMaxRow;
+}
+
+// `order` = the helper engine's matrices, then the one-workgroup engine's, each largest first; uploads it and
+// its block map.
+int batch_partition(rcm_batch* b) {
+  const int32_t C = b->count;
+  std::vector<int32_t> by_size((size_t)C);
+  for (int32_t k = 0; k < C; ++k) by_size[k] = k;
+  std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t x, int32_t y) { return b->nnz[x] > b->nnz[y]; });
+  b->wg.assign((size_t)C, 0);
+  b->order.clear();
+  for (int32_t k : by_size)
+    if (!wg_eligible(b, k)) b->order.push_back(k);
+  b->n_helper = (int32_t)b->order.size();
+  for (int32_t k : by_size)
+    if (wg_eligible(b, k)) {
+      b->order.push_back(k);
+      b->wg[k] = 1;
+    }
+  b->nblk_order = build_blk(b, b->order.data(), C, b->blk_order_h);
+  RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->blk_order, b->blk_order_h.data(), ((size_t)C + 1) * sizeof(int32_t),
+                         hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipStreamSynchronize(b->stream));
+  return RCM_OK;
 }
 
 // Per-matrix status from the control blocks of the last launch (jobs listed in `order`).
@@ -1082,6 +1137,16 @@ int rcm_batch_create(int device, rcm_batch** out) {
     rcm_batch_destroy(b);
     return RCM_EDEVICE;
   }
+  if (ok) {
+    int per_cu = 0;
+    ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_order_wg<false>),
+                                                      kWgT, 0) == hipSuccess;
+    b->max_res_wg = std::max(1, per_cu) * prop.multiProcessorCount;
+  }
+  if (!ok) {
+    rcm_batch_destroy(b);
+    return RCM_EDEVICE;
+  }
   b->max_resident = b->max_res[0];
   *out = b;
   return RCM_OK;
@@ -1111,6 +1176,11 @@ int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
       batch_drop_graphs(b);
       return RCM_OK;
     case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_WG_MAX_N:
+      if (v < 0) return RCM_EINVAL;
+      b->wg_max_n = v;
+      batch_drop_graphs(b);
+      return b->count > 0 ? batch_partition(b) : RCM_OK;
     case RCM_PARAM_VARIANT:
       if (v < 0 || v >= kVariants) return RCM_EINVAL;
       b->variant = v;
@@ -1209,11 +1279,14 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   RCM_HIP(hipMemcpyAsync(v.data(), b->scratch, 2 * C * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
   RCM_HIP(hipStreamSynchronize(b->stream));
   int rc = RCM_OK;
+  b->maxrow.assign(C, 0);
   for (int32_t k = 0; k < count; ++k) {
     const int r = v[2 * k] ? RCM_EINVAL : RCM_OK;
     if (status_out) status_out[k] = r;
     if (r) rc = r;
+    b->maxrow[k] = v[2 * k + 1];
   }
+  if (rc == RCM_OK) rc = batch_partition(b);
   b->timing = rcm_batch_timing_t{};
   b->timing.count = count;
   b->timing.total_n = tot;
@@ -1250,10 +1323,12 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
     RCM_HIP(hipMemcpyAsync(b->d_starts, cur.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
     RCM_HIP(hipMemcpyAsync(b->d_active, active.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
     std::vector<int32_t> blk;
-    const int32_t nblk = build_blk(b, active.data(), A, blk);
+    (void)build_blk(b, active.data(), A, blk);
     RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
                            b->stream));
-    enqueue_many(b, b->d_active, b->blk_active, nblk, A, true);
+    int32_t nh = 0;  // `active` keeps the order of `order`: the helper engine's matrices come first
+    while (nh < A && !b->wg[active[nh]]) ++nh;
+    enqueue_many(b, b->d_active, b->blk_active, blk[nh], nh, A - nh, true);
     hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
     RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
     RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
@@ -1310,7 +1385,7 @@ int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* s
   RCM_HIP(hipMemcpyAsync(b->d_starts, start, (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipEventRecord(b->ev[0], b->stream));
   auto enqueue_run = [&]() {
-    enqueue_many(b, b->d_order, b->blk_order, b->nblk_order, C, false);
+    enqueue_many(b, b->d_order, b->blk_order, b->blk_order_h[b->n_helper], b->n_helper, C - b->n_helper, false);
     if (rev)
       hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
                          C, b->blk_order);

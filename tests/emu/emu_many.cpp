@@ -2,6 +2,7 @@
 // TEST INFRASTRUCTURE ONLY. Built by tests/emu/Makefile into tests/emu/libemu_many.so.
 #include "wave_emu.hpp"
 #include "engine_many.hpp"
+#include "engine_wg.hpp"
 
 #include <vector>
 
@@ -79,5 +80,62 @@ int emu_run_many(int32_t count, const int32_t* n, const int64_t* nnz, const int6
     info[4 * k + 3] = (int32_t)ctl[k].done;
   }
   return (int)g.jobs_done;
+}
+
+// One workgroup per matrix (engine_wg.hpp) over `count` concatenated matrices, `workers` persistent
+// workgroups. perm_cat / level_cat as emu_run_many; info[2*k] = placed, info[2*k+1] = levels.
+int emu_run_wg(int32_t count, const int32_t* n, const int64_t* rp_off, const int64_t* ci_off,
+               const int32_t* row_ptr_cat, const int32_t* col_idx_cat, const int32_t* start, int32_t workers,
+               int32_t bfs, int32_t* perm_cat, int32_t* level_cat, int32_t* info) {
+  if (count <= 0 || workers <= 0) return -1;
+  int64_t tot = 0;
+  for (int k = 0; k < count; ++k) {
+    int32_t maxrow = 0;
+    const int32_t* rp = row_ptr_cat + rp_off[k];
+    for (int32_t i = 0; i < n[k]; ++i) maxrow = std::max(maxrow, rp[i + 1] - rp[i]);
+    if (maxrow > kWgMaxRow) return -2;
+    tot += n[k];
+  }
+  std::vector<int32_t> perm(tot, -1), level(tot, -1);
+  std::vector<uint64_t> claim(tot, 0x5A5A5A5A5A5A5A5Aull), qinfo(tot);
+  std::vector<Control> ctl(count);
+  std::vector<JobDesc> jobs(count);
+  std::vector<int32_t> order(count), starts(start, start + count);
+  int64_t po = 0;
+  for (int k = 0; k < count; ++k) {
+    JobDesc& J = jobs[k];
+    J = JobDesc{};
+    J.row_ptr = row_ptr_cat + rp_off[k];
+    J.col_idx = col_idx_cat + ci_off[k];
+    J.perm = perm.data() + po;
+    J.qinfo = qinfo.data() + po;
+    J.claim = claim.data() + po;
+    J.level = level.data() + po;
+    J.ctl = &ctl[k];
+    J.n = n[k];
+    po += n[k];
+    order[k] = count - 1 - k;
+  }
+  ManyControl g{};
+  ManyArgs m{};
+  m.jobs = jobs.data();
+  m.order = order.data();
+  m.starts = starts.data();
+  m.count = count;
+  m.g = &g;
+  emu::launch((unsigned)((count + 255) / 256), 256, false, k_wg_prepare, m);
+  emu::emu_lds_bytes = sizeof(WgSmem);
+  if (bfs)
+    emu::launch((unsigned)workers, kWgT, true, k_order_wg<true>, m);
+  else
+    emu::launch((unsigned)workers, kWgT, true, k_order_wg<false>, m);
+  emu::emu_lds_bytes = 256 * 1024;
+  std::memcpy(perm_cat, perm.data(), sizeof(int32_t) * tot);
+  if (level_cat) std::memcpy(level_cat, level.data(), sizeof(int32_t) * tot);
+  for (int k = 0; k < count; ++k) {
+    info[2 * k] = (int32_t)ctl[k].placed;
+    info[2 * k + 1] = (int32_t)ctl[k].nbatches;
+  }
+  return 0;
 }
 }

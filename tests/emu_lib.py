@@ -172,3 +172,30 @@ def level_order(rp, ci, start, bfs=False):
                                     C.byref(placed), C.byref(levels))
     k = placed.value
     return rc, perm[:k], (lev[:k] if bfs else None), levels.value
+
+
+def run_wg(mats, starts, workers=3, bfs=False):
+    """One workgroup per matrix (engine_wg.hpp). mats: list of (rp, ci). Returns (rc, [perm], [level], info)."""
+    L = _many_lib()
+    if not hasattr(L, "_wg_ready"):
+        I64P = C.POINTER(C.c_int64)
+        L.emu_run_wg.restype = C.c_int
+        L.emu_run_wg.argtypes = [C.c_int32, I32P, I64P, I64P, I32P, I32P, I32P, C.c_int32, C.c_int32, I32P, I32P,
+                                 I32P]
+        L._wg_ready = True
+    n = np.array([rp.shape[0] - 1 for rp, _ in mats], np.int32)
+    rp_off = np.concatenate([[0], np.cumsum([rp.shape[0] for rp, _ in mats])[:-1]]).astype(np.int64)
+    ci_off = np.concatenate([[0], np.cumsum([ci.shape[0] for _, ci in mats])[:-1]]).astype(np.int64)
+    rpc = np.ascontiguousarray(np.concatenate([rp for rp, _ in mats]), np.int32)
+    cic = np.ascontiguousarray(np.concatenate([ci for _, ci in mats]), np.int32)
+    st = np.ascontiguousarray(starts, np.int32)
+    tot = int(n.sum())
+    perm = np.full(tot, -1, np.int32)
+    lev = np.full(tot, -1, np.int32)
+    info = np.zeros(2 * len(mats), np.int32)
+    I64 = C.POINTER(C.c_int64)
+    rc = L.emu_run_wg(len(mats), n.ctypes.data_as(I32P), rp_off.ctypes.data_as(I64), ci_off.ctypes.data_as(I64),
+                      rpc.ctypes.data_as(I32P), cic.ctypes.data_as(I32P), st.ctypes.data_as(I32P), workers,
+                      1 if bfs else 0, perm.ctypes.data_as(I32P), lev.ctypes.data_as(I32P), info.ctypes.data_as(I32P))
+    off = np.concatenate([[0], np.cumsum(n)])
+    return rc, [perm[off[k]:off[k + 1]] for k in range(len(mats))], [lev[off[k]:off[k + 1]] for k in range(len(mats))], info

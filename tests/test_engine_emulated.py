@@ -185,3 +185,26 @@ def test_engine_variants_long_rows(slots):
     rp, ci = LONG["hub5000"]()
     perm, _, info = emu_lib.run(rp, ci, 3, workers=2, slots=slots, engine="block", timeout_s=60)
     assert info[0] == 0 and np.array_equal(perm, O.cm(rp, ci, 3))
+
+
+def test_one_workgroup_engine_matches_oracle():
+    """engine_wg.hpp (one workgroup per matrix, level by level) in the emulator: CM and plain-BFS orders of a
+    mixed set from several starts, including duplicates, unsorted rows, a 1-node and a 2-node graph."""
+    mats = [G.path(17), G.star(40), G.complete(9), G.grid2d(7, 5), G.random_connected(500, 1500, seed=4),
+            G.with_duplicates(*G.random_connected(300, 600, seed=5), seed=6),
+            G.shuffle_rows(*G.random_connected(300, 900, seed=7), seed=8), G.path(1), G.path(2)]
+    m = P.workloads.config(3, scale=3000, seed=5)
+    mats.append((m.row_offsets, m.col_ids))
+    for pick in (lambda n: n // 2, lambda n: 0, lambda n: n - 1):
+        starts = [pick(rp.shape[0] - 1) for rp, _ in mats]
+        for bfs in (False, True):
+            rc, perms, levs, info = emu_lib.run_wg(mats, starts, workers=3, bfs=bfs)
+            assert rc == 0
+            for k, (rp, ci) in enumerate(mats):
+                if bfs:
+                    order, lev = O.bfs(rp, ci, starts[k])
+                    np.testing.assert_array_equal(perms[k], order)
+                    np.testing.assert_array_equal(levs[k], lev)
+                else:
+                    np.testing.assert_array_equal(perms[k], O.cm(rp, ci, starts[k]))
+                assert info[2 * k] == rp.shape[0] - 1
