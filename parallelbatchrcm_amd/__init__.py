@@ -609,6 +609,7 @@ class RcmContext:
         _check(self._lib.rcm_ctx_profile(self._h, buf, 16), "profile")
         out = {ph: buf[i] * 0.01 for i, ph in enumerate(PROFILE_PHASES)}  # 100 MHz ticks -> us
         out.update(batches=int(buf[8]), early=int(buf[9]), children=int(buf[10]), slots=int(buf[11]))
+        out["compact"] = buf[12] * 0.01  # compaction, before "sort" (the ordering itself)
         return out
 
     def timing(self) -> Timing:

@@ -105,6 +105,7 @@ enum : int {
   PF_EARLY = 9,     // count of early batches
   PF_CHILDREN = 10, // sum of owned children
   PF_SLOTS = 11,    // sum of adjacency slots
+  PF_COMPACT = 12,  // compaction of owned children (PF_SORT is the ordering after it)
   kProfSlots = 16
 };
 
@@ -132,10 +133,13 @@ struct Smem {
   uint16_t si[G::S];   // parent slot of each adjacency slot
   uint64_t key[2][G::S];
   uint16_t val[2][G::S];
-  uint32_t hist[HS];
+  uint32_t hist[HS];  // radix histogram; the segment sort keeps its segment starts and degree bases here
+  uint32_t cinfo[2 * (G::S / kWave)];  // per 64-slot chunk: count, degree sum (compaction and cut exchanges)
   uint32_t wtmp[2 * G::W + 2];
   int32_t b, s, e, early, quit;
   uint32_t cnt, nb, excl_off, excl_nb, maxdeg, maxpdeg;
+  static_assert(HS >= 2 * G::P, "segment starts and degree bases of every parent fit in hist");
+  static_assert(G::S == G::W * kWave * G::EPT, "compaction rounds cover the slots");
 };
 
 __device__ __forceinline__ void raise_error(const RunArgs& a, uint32_t err, uint32_t site = 0, uint32_t v0 = 0,
@@ -321,24 +325,37 @@ __device__ int lds_radix_sort(Smem<G, DIGIT>& sm, uint32_t cnt, uint32_t dbits, 
   return cur;
 }
 
-// ---- stable segment-rank sort --------------------------------------------------------------------------
-// Compacted elements of one parent are contiguous (compaction keeps slot order). Each element counts the
-// elements of its own segment that order before it by (degree, position) and moves to segment start + rank.
+// ---- stable segment-rank sort, with the degree prefix --------------------------------------------------------
+// Compacted elements of one parent are contiguous (compaction keeps slot order): segment `seg` starts at
+// hist[seg], and its children's degrees before it sum to hist[P + seg]. Each element reads its segment's keys
+// U at a time (independent loads), counts those that order before it by (degree, position) and sums their
+// degrees, then moves to segment start + rank with the key (exclusive degree prefix in sorted order << 32) |
+// degree.
 template <class G, int DIGIT>
 __device__ int lds_segment_sort(Smem<G, DIGIT>& sm, uint32_t cnt) {
-  const int tid = threadIdx.x;
-  for (uint32_t e = tid; e < cnt; e += G::T) {
+  constexpr uint32_t U = 4;
+  for (uint32_t e = threadIdx.x; e < cnt; e += G::T) {
     const uint64_t k = sm.key[0][e];
-    const uint32_t seg = (uint32_t)(k >> 32);
-    uint32_t lo = e;
-    while (lo > 0 && (uint32_t)(sm.key[0][lo - 1] >> 32) == seg) --lo;
-    uint32_t rank = 0;
-    for (uint32_t q = lo; q < cnt; ++q) {
-      const uint64_t kq = sm.key[0][q];
-      if ((uint32_t)(kq >> 32) != seg) break;
-      rank += (kq < k || (kq == k && q < e)) ? 1u : 0u;
+    const uint32_t seg = (uint32_t)(k >> 32), deg = (uint32_t)k;
+    const uint32_t lo = sm.hist[seg];
+    uint32_t rank = 0, dsum = 0;
+    for (uint32_t q0 = lo;; q0 += U) {
+      uint64_t kq[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) kq[u] = sm.key[0][min(q0 + u, cnt - 1u)];  // in range, so they issue together
+      bool all_same = true;
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t q = q0 + u, dq = (uint32_t)kq[u];
+        const bool same = q < cnt && (uint32_t)(kq[u] >> 32) == seg;
+        const bool before = same && (dq < deg || (dq == deg && q < e));
+        rank += before ? 1u : 0u;
+        dsum += before ? dq : 0u;
+        all_same = all_same && same;
+      }
+      if (!all_same) break;  // the segment ended inside this group (segments are contiguous)
     }
-    sm.key[1][lo + rank] = k;
+    sm.key[1][lo + rank] = ((uint64_t)(sm.hist[G::P + seg] + dsum) << 32) | (uint64_t)deg;
     sm.val[1][lo + rank] = sm.val[0][e];
   }
   __syncthreads();
@@ -645,7 +662,7 @@ __device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_
 // ---- one batch ------------------------------------------------------------------------------------------
 template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF>
 __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline, uint64_t* pf) {
-  constexpr int T = G::T, S = G::S, EPT = G::EPT;
+  constexpr int T = G::T, S = G::S, EPT = G::EPT, W = G::W;
   const int tid = threadIdx.x;
   const uint32_t b = (uint32_t)sm.b;
   const int32_t s = sm.s, e = sm.e;
@@ -794,87 +811,176 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   }
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_FINAL] += _t - pf[15]; pf[15] = _t; } }
-  // C. compact owned children in slot order (blocked: thread t scans slots [t*EPT, t*EPT+EPT))
-  uint32_t lc = 0, lmax = 0;
+  // C. compact owned children in slot order. The slots form chunks of 64 (chunk c: slots [64c, 64c + 64)); wave w
+  //    takes chunks w, w + W, w + 2W, ... below D (lane l: slot 64c + l), so the LDS reads are free of bank
+  //    conflicts and a short batch costs every wave only the chunks it has. Ballots rank owned slots within a
+  //    chunk; one exchange of per-chunk counts and degree sums, scanned by every wave, gives each chunk its base.
+  //    Keys: plain BFS (kept in slot order): (exclusive degree prefix << 32) | degree; CM: (parent slot << 32) |
+  //    degree. For the segment sort, hist[i] and hist[P + i] receive parent i's segment start and degree base.
+  constexpr int NC = S / kWave;
+  static_assert(NC <= kWave, "one wave scans the chunk counts");
+  const bool seg_sort =
+      wave_uniform(!BFS && (a.sort_mode == 2 || (a.sort_mode == 0 && maxpdeg <= (uint32_t)a.seg_threshold)) ? 1u : 0u);
+  const bool want_dx = BFS || seg_sort;
+  const int wv = (int)wave_uniform((uint32_t)wave_id()), ln = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t Du = wave_uniform(D);
+  const uint32_t nch = (Du + kWave - 1) / kWave;
+  uint64_t obal[EPT];
+  uint32_t sdeg[EPT], dexc[EPT];
+  uint32_t lmax = 0;
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const uint32_t j = (uint32_t)(tid * EPT + k);
-    if (j < D && sm.sc[j] >= 0) {
-      ++lc;
-      lmax = max(lmax, sm.sre[j] - sm.srb[j]);
+    const uint32_t c = (uint32_t)(k * W + wv);
+    obal[k] = 0;
+    sdeg[k] = 0;
+    dexc[k] = 0;
+    if (c < nch) {
+      const uint32_t j = c * kWave + (uint32_t)ln;
+      const bool own = j < Du && sm.sc[j] >= 0;
+      sdeg[k] = own ? sm.sre[j] - sm.srb[j] : 0u;
+      lmax = max(lmax, sdeg[k]);
+      obal[k] = ballot(own);
+      uint32_t csum = 0;
+      if (want_dx) {
+        const uint32_t inc = wave_incl_scan(sdeg[k]);
+        dexc[k] = inc - sdeg[k];
+        csum = lane63(inc);
+      }
+      if (ln == 0) {
+        sm.cinfo[c] = (uint32_t)__popcll(obal[k]);
+        sm.cinfo[NC + c] = csum;
+      }
     }
   }
-  uint32_t cnt, maxdeg;
-  uint32_t pos = block_excl_scan_max<T>(lc, lmax, sm.wtmp, &cnt, &maxdeg);
+  {
+    const uint32_t wm = wave_max(lmax);
+    if (ln == 0) sm.wtmp[wv] = wm;
+  }
+  __syncthreads();
+  // (no second barrier: cinfo and wtmp are next written after the barrier that ends this phase)
+  uint32_t cnt, maxdeg = 0, cex, dex;
+  {
+    const uint32_t cc = (uint32_t)ln < nch ? sm.cinfo[ln] : 0u, cd = (uint32_t)ln < nch ? sm.cinfo[NC + ln] : 0u;
+    const uint32_t ci = wave_incl_scan(cc), di = wave_incl_scan(cd);
+    cnt = lane63(ci);
+    cex = ci - cc;  // lane l: owned slots before chunk l
+    dex = di - cd;  // lane l: their degree sum
+#pragma unroll
+    for (int w2 = 0; w2 < W; ++w2) maxdeg = max(maxdeg, sm.wtmp[w2]);
+  }
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const uint32_t j = (uint32_t)(tid * EPT + k);
-    if (j < D && sm.sc[j] >= 0) {
-      sm.key[0][pos] = ((uint64_t)sm.si[j] << 32) | (uint64_t)(sm.sre[j] - sm.srb[j]);
-      sm.val[0][pos] = (uint16_t)j;
-      ++pos;
+    const uint32_t c = (uint32_t)(k * W + wv);
+    if (c < nch) {
+      const uint32_t pbase = (uint32_t)__builtin_amdgcn_readlane((int)cex, (int)c);
+      const uint32_t dbase = (uint32_t)__builtin_amdgcn_readlane((int)dex, (int)c);
+      const uint32_t j = c * kWave + (uint32_t)ln;
+      const uint32_t p = pbase + (uint32_t)__popcll(obal[k] & lt);  // owned slots before slot j
+      if (j < Du) {
+        const uint32_t i = sm.si[j];
+        if (seg_sort && (j == 0 || sm.si[j - 1] != i)) {  // first slot of parent i
+          sm.hist[i] = p;
+          sm.hist[G::P + i] = dbase + dexc[k];
+        }
+        if ((obal[k] >> ln) & 1ull) {
+          sm.key[0][p] = ((uint64_t)(BFS ? dbase + dexc[k] : i) << 32) | (uint64_t)sdeg[k];
+          sm.val[0][p] = (uint16_t)j;
+        }
+      }
     }
   }
   __syncthreads();
 
-  // D. order: plain BFS keeps slot order; CM sorts by (parent slot, degree), stably
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_COMPACT] += _t - pf[15]; pf[15] = _t; } }
+  // D. order: plain BFS keeps slot order; CM sorts by (parent slot, degree), stably. dx_key: the keys of the
+  //    final order carry the exclusive degree prefix in their high word.
   int fin = 0;
+  bool dx_key = BFS;
   if constexpr (!BFS) {
     if (cnt > 1) {
-      const bool seg = a.sort_mode == 2 || (a.sort_mode == 0 && maxpdeg <= (uint32_t)a.seg_threshold);
-      if (seg)
+      if (seg_sort) {
         fin = lds_segment_sort<G, DIGIT>(sm, cnt);
-      else
+        dx_key = true;
+      } else {
         fin = lds_radix_sort<G, DIGIT>(sm, cnt, bits_for(maxdeg), bits_for((uint32_t)(m - 1)));
+      }
     }
   }
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_SORT] += _t - pf[15]; pf[15] = _t; } }
   // E. cut into child batches: a child starts a new batch at a count quantum (P), at a degree quantum (Q),
   //    or next to a child of degree > Q; every batch then has <= P parents and <= S slots or one node.
+  //    Positions are taken in 64-entry chunks like the slots in C; a child's degree is its key's low word, its
+  //    exclusive degree prefix the key's high word (dx_key) or a scan here.
   uint64_t* cut = sm.key[fin ^ 1];
-  uint32_t ldeg[EPT];
-  uint32_t lsum = 0;
+  const uint32_t nco = (cnt + kWave - 1) / kWave;
+  uint32_t ldeg[EPT], ldx[EPT], pdeg[EPT];
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const uint32_t o = (uint32_t)(tid * EPT + k);
+    const uint32_t c = (uint32_t)(k * W + wv);
     ldeg[k] = 0;
-    if (o < cnt) {
-      const uint32_t j = sm.val[fin][o];
-      ldeg[k] = sm.sre[j] - sm.srb[j];
+    ldx[k] = 0;
+    pdeg[k] = 0;
+    if (c < nco) {
+      const uint32_t o = c * kWave + (uint32_t)ln;
+      const uint64_t w = o < cnt ? sm.key[fin][o] : 0ull;
+      const uint64_t pw = (o > 0 && o < cnt) ? sm.key[fin][o - 1] : 0ull;
+      ldeg[k] = (uint32_t)w;
+      ldx[k] = (uint32_t)(w >> 32);
+      pdeg[k] = (uint32_t)pw;
     }
-    lsum += ldeg[k];
   }
-  uint32_t dtot;
-  uint32_t dx = block_excl_scan<T>(lsum, sm.wtmp, &dtot);
-  uint32_t lstarts = 0;
-  uint32_t sflag = 0;  // bit k: element tid*EPT+k starts a batch
-  {
-    uint32_t prev_dx = 0, prev_deg = 0;
-    const uint32_t o0 = (uint32_t)(tid * EPT);
-    if (o0 > 0 && o0 < cnt) {
-      const uint32_t jp = sm.val[fin][o0 - 1];
-      prev_deg = sm.sre[jp] - sm.srb[jp];
-      prev_dx = dx - prev_deg;
-    }
+  if (!dx_key) {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-      const uint32_t o = o0 + (uint32_t)k;
-      if (o < cnt) {
-        const bool st = o == 0 || (dx / G::Q) != (prev_dx / G::Q) || (o / G::P) != ((o - 1) / G::P) ||
-                        ldeg[k] > G::Q || prev_deg > G::Q;
-        if (st) { sflag |= 1u << k; ++lstarts; }
-        prev_dx = dx;
-        prev_deg = ldeg[k];
-        dx += ldeg[k];
+      const uint32_t c = (uint32_t)(k * W + wv);
+      if (c < nco) {
+        const uint32_t inc = wave_incl_scan(ldeg[k]);
+        ldx[k] = inc - ldeg[k];
+        const uint32_t tot = lane63(inc);
+        if (ln == 0) sm.cinfo[NC + c] = tot;
+      }
+    }
+    __syncthreads();
+    const uint32_t cd = (uint32_t)ln < nco ? sm.cinfo[NC + ln] : 0u;
+    const uint32_t dexo = wave_incl_scan(cd) - cd;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t c = (uint32_t)(k * W + wv);
+      if (c < nco) ldx[k] += (uint32_t)__builtin_amdgcn_readlane((int)dexo, (int)c);
+    }
+  }
+  uint64_t sbal[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint32_t c = (uint32_t)(k * W + wv);
+    sbal[k] = 0;
+    if (c < nco) {
+      const uint32_t o = c * kWave + (uint32_t)ln;
+      const uint32_t prev_dx = ldx[k] - pdeg[k];  // the previous child's prefix
+      const bool st = o < cnt && (o == 0 || (ldx[k] / G::Q) != (prev_dx / G::Q) || (o / G::P) != ((o - 1) / G::P) ||
+                                  ldeg[k] > G::Q || pdeg[k] > G::Q);
+      sbal[k] = ballot(st);
+      if (ln == 0) sm.cinfo[c] = (uint32_t)__popcll(sbal[k]);
+    }
+  }
+  __syncthreads();
+  uint32_t nb;
+  {
+    const uint32_t cs = (uint32_t)ln < nco ? sm.cinfo[ln] : 0u;
+    const uint32_t si = wave_incl_scan(cs);
+    nb = lane63(si);
+    const uint32_t sex = si - cs;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint32_t c = (uint32_t)(k * W + wv);
+      if (c < nco) {
+        const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)sex, (int)c);  // starts before chunk c
+        if ((sbal[k] >> ln) & 1ull) cut[cb + (uint32_t)__popcll(sbal[k] & lt)] = (uint64_t)(c * kWave + (uint32_t)ln);
       }
     }
   }
-  uint32_t nb;
-  uint32_t bidx = block_excl_scan<T>(lstarts, sm.wtmp, &nb);
-#pragma unroll
-  for (int k = 0; k < EPT; ++k)
-    if (sflag & (1u << k)) cut[bidx++] = (uint64_t)(tid * EPT + k);
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CUT] += _t - pf[15]; pf[15] = _t; } }
   // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record
@@ -986,7 +1092,7 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   }
   if constexpr (PROF) {
     if (tid == 0 && a.prof)
-      for (int i = 0; i < 12; ++i) atomic_add_u64(&a.prof[i], pf[i]);
+      for (int i = 0; i <= PF_COMPACT; ++i) atomic_add_u64(&a.prof[i], pf[i]);
   }
 }
 
