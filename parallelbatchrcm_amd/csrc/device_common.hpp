@@ -91,6 +91,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   v += dpp_move<0x143, 0xc>(v, 0u);
   return v;
 }
+// Inclusive prefix maximum over the wave (the same row moves; 0 is the identity for unsigned values).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  v = max(v, dpp_move<0x111>(v, 0u));
+  v = max(v, dpp_move<0x112>(v, 0u));
+  v = max(v, dpp_move<0x114>(v, 0u));
+  v = max(v, dpp_move<0x118>(v, 0u));
+  v = max(v, dpp_move<0x142, 0xa>(v, 0u));
+  v = max(v, dpp_move<0x143, 0xc>(v, 0u));
+  return v;
+}
 // Reductions: the same prefix pattern with max / min; lane 63 holds the result, returned to every lane.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return lane63(wave_incl_scan(v)); }
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {

@@ -130,7 +130,8 @@ struct Smem {
   int32_t sc[G::S];    // child id, -1 when not owned
   uint32_t srb[G::S];  // child row begin
   uint32_t sre[G::S];  // child row end
-  uint16_t si[G::S];   // parent slot of each adjacency slot
+  alignas(16) uint16_t si[G::S];  // parent slot of each adjacency slot (before the claims: parent marks)
+  uint16_t cpar[G::S / kWave];    // per 64-slot chunk: 1 + the parent slot owning the chunk's first slot
   uint64_t key[2][G::S];
   uint16_t val[2][G::S];
   uint32_t hist[HS];  // radix histogram; the segment sort keeps its segment starts and degree bases here
@@ -232,17 +233,6 @@ __device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uin
     if (should_quit(a, deadline)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
-}
-
-// Largest i in [0, m) with so[i] <= j (so[0] == 0 <= j).
-__device__ __forceinline__ int parent_of_slot(const uint32_t* so, int m, uint32_t j) {
-  int lo = 0, hi = m;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (so[mid] <= j) lo = mid;
-    else hi = mid;
-  }
-  return lo;
 }
 
 // ---- stable LDS radix sort of (key, val) pairs, cnt <= S ---------------------------------------------
@@ -688,10 +678,27 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     const bool ok = claims_complete_before(a, (int64_t)b);
     if (lane_id() == 0) sm.early = ok ? 1 : 0;
   }
+  // Slot -> parent marks for the claims: si[first slot of parent i] = i + 1, 0 on every other slot, and for each
+  // 64-slot chunk the parent (+ 1) owning its first slot; a prefix maximum over a chunk's marks then gives every
+  // slot its parent. The clear is ordered before the marks by the scan's barriers.
+  {
+    uint32_t* si32 = reinterpret_cast<uint32_t*>(sm.si);
+    for (int q = tid * 4; q < S / 2; q += T * 4) {
+      si32[q] = 0u;
+      si32[q + 1] = 0u;
+      si32[q + 2] = 0u;
+      si32[q + 3] = 0u;
+    }
+  }
   uint32_t D, maxpdeg;
   const uint32_t soff = block_excl_scan_max<T>(d, d, sm.wtmp, &D, &maxpdeg);
   if (tid < m) sm.so[tid] = soff;
   if (tid == 0) sm.so[m] = D;
+  if (tid < m && d > 0) {
+    if (soff < (uint32_t)S) sm.si[soff] = (uint16_t)(tid + 1);
+    for (uint32_t c = (soff + kWave - 1) / kWave; c < (uint32_t)(S / kWave) && c * kWave < soff + d; ++c)
+      sm.cpar[c] = (uint16_t)(tid + 1);
+  }
   __syncthreads();
   if (D > (uint32_t)S) {
     // only a batch of one node can exceed the slots (the cut rule keeps multi-node batches below S)
@@ -707,24 +714,33 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   const bool early = sm.early != 0;
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PARENTS] += _t - pf[15]; pf[15] = _t; } }
-  // B. claim every adjacency slot (striped: thread t owns slots t, t+T, ...). Staged so that all column
-  //    loads, then all atomics and row-bound loads of a thread are in flight together.
+  // B. claim every adjacency slot (striped: thread t owns slots t, t+T, ..., so round k of wave w is the 64-slot
+  //    chunk kW + w). Staged so that all column loads, then all atomics and row-bound loads of a thread are in
+  //    flight together. A slot's parent is the prefix maximum of the chunk's parent marks.
   uint64_t code[EPT], oldv[EPT];
   int32_t cc[EPT];
   uint32_t crb[EPT], cre[EPT];
   uint16_t ci16[EPT];
+  const uint32_t n_chunks = (wave_uniform(D) + kWave - 1) / kWave;
+  const uint32_t wave0 = wave_uniform((uint32_t)wave_id());
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     code[k] = 0;
     cc[k] = 0;
     ci16[k] = 0;
-    const uint32_t j = (uint32_t)(tid + k * T);
-    if (j < D) {
-      const int i = parent_of_slot(sm.so, m, j);
-      const uint32_t r = j - sm.so[i];
-      cc[k] = a.col_idx[sm.prb[i] + r];
-      code[k] = ((uint64_t)(uint32_t)(s + i) << 32) | (uint64_t)r;
-      ci16[k] = (uint16_t)i;
+    const uint32_t c = (uint32_t)(k * (T / kWave)) + wave0;
+    if (c < n_chunks) {
+      const uint32_t j = (uint32_t)(tid + k * T);
+      uint32_t mk = j < D ? (uint32_t)sm.si[j] : 0u;
+      if (lane_id() == 0) mk = max(mk, (uint32_t)sm.cpar[c]);
+      mk = wave_incl_max(mk);
+      if (j < D) {
+        const uint32_t i = mk - 1u;
+        const uint32_t r = j - sm.so[i];
+        cc[k] = a.col_idx[sm.prb[i] + r];
+        code[k] = ((uint64_t)(uint32_t)(s + (int32_t)i) << 32) | (uint64_t)r;
+        ci16[k] = (uint16_t)i;
+      }
     }
   }
   // One wait for every column load: afterwards the atomics below issue back to back (without it the
