@@ -789,6 +789,11 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   __syncthreads();
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CLAIM] += _t - pf[15]; pf[15] = _t; } }
+  // won[k]: slot tid + kT owns its child. Decided in registers on the usual path, from the LDS words after the
+  // in-batch correction on the early one.
+  bool won[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) won[k] = (uint32_t)(tid + k * T) < D && oldv[k] > code[k];
   if (early) {
     // every earlier batch had finished claiming: a won atomicMin is final unless a slot of this same batch
     // with a smaller code took the node afterwards; that slot saw our code as its old value.
@@ -804,6 +809,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     publish_wait<STRICT>();
     __syncthreads();
     if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DINC, 0, 0));
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) won[k] = won[k] && sm.sc[tid + k * T] >= 0;
   } else {
     publish_wait<STRICT>();
     __syncthreads();
@@ -816,21 +823,19 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     if (sm.quit) return;
     consume_fence<STRICT>();
     if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_DINC, 0, 0));
+    // every earlier batch has claimed: the child is ours iff its claim word still holds our code (the LDS
+    // slot word of a lost child is not read again: compaction takes won[], the writes only owned slots)
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const uint32_t j = (uint32_t)(tid + k * T);
-      if (j < D && sm.sc[j] >= 0) {
-        if (ld_sync(&a.claim[sm.sc[j]]) != code[k]) sm.sc[j] = -1;
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < EPT; ++k)
+      if (won[k]) won[k] = ld_sync(&a.claim[cc[k]]) == code[k];
   }
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_FINAL] += _t - pf[15]; pf[15] = _t; } }
   // C. compact owned children in slot order. The slots form chunks of 64 (chunk c: slots [64c, 64c + 64)); wave w
-  //    takes chunks w, w + W, w + 2W, ... below D (lane l: slot 64c + l), so the LDS reads are free of bank
-  //    conflicts and a short batch costs every wave only the chunks it has. Ballots rank owned slots within a
-  //    chunk; one exchange of per-chunk counts and degree sums, scanned by every wave, gives each chunk its base.
+  //    holds chunks w, w + W, w + 2W, ... (lane l: slot 64c + l, the claim's striping) with each slot's ownership,
+  //    degree, parent and row entry in registers, and skips the chunks at or beyond D. Ballots rank owned slots
+  //    within a chunk; one exchange of per-chunk counts and degree sums, scanned by every wave, gives each chunk
+  //    its base.
   //    Keys: plain BFS (kept in slot order): (exclusive degree prefix << 32) | degree; CM: (parent slot << 32) |
   //    degree. For the segment sort, hist[i] and hist[P + i] receive parent i's segment start and degree base.
   constexpr int NC = S / kWave;
@@ -852,9 +857,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     sdeg[k] = 0;
     dexc[k] = 0;
     if (c < nch) {
-      const uint32_t j = c * kWave + (uint32_t)ln;
-      const bool own = j < Du && sm.sc[j] >= 0;
-      sdeg[k] = own ? sm.sre[j] - sm.srb[j] : 0u;
+      const bool own = won[k];
+      sdeg[k] = own ? cre[k] - crb[k] : 0u;
       lmax = max(lmax, sdeg[k]);
       obal[k] = ballot(own);
       uint32_t csum = 0;
@@ -894,8 +898,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       const uint32_t j = c * kWave + (uint32_t)ln;
       const uint32_t p = pbase + (uint32_t)__popcll(obal[k] & lt);  // owned slots before slot j
       if (j < Du) {
-        const uint32_t i = sm.si[j];
-        if (seg_sort && (j == 0 || sm.si[j - 1] != i)) {  // first slot of parent i
+        const uint32_t i = ci16[k];
+        if (seg_sort && (uint32_t)code[k] == 0u) {  // row entry 0: the first slot of parent i
           sm.hist[i] = p;
           sm.hist[G::P + i] = dbase + dexc[k];
         }
