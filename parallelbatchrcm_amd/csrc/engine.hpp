@@ -193,11 +193,15 @@ __device__ __forceinline__ bool claims_complete_before(const RunArgs& a, int64_t
   }
 }
 
+// Waits check the run's done word and the watchdog on every kQuitPoll-th poll only, so a poll is one memory
+// round trip instead of two; a stop is seen at most kQuitPoll polls late.
+constexpr uint32_t kQuitPoll = 8;
+
 // Blocking form; returns false if the run stopped while waiting.
 __device__ __forceinline__ bool wait_claims_complete(const RunArgs& a, int64_t b, uint64_t deadline) {
-  for (uint32_t spin = 0;; ++spin) {
+  for (uint32_t spin = 1;; ++spin) {
     if (claims_complete_before(a, b)) return true;
-    if (should_quit(a, deadline)) return false;
+    if (spin % kQuitPoll == 0 && should_quit(a, deadline)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
 }
@@ -208,7 +212,7 @@ __device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uin
                                                 uint32_t* nbid) {
   int64_t base = b - 1;
   uint32_t acc_off = 0, acc_nb = 0;
-  for (;;) {
+  for (uint32_t spin = 1;;) {
     const int64_t idx = base - lane_id();
     const uint64_t w = idx >= 0 ? ld_sync(&a.bstat[idx]) : mk_status(ST_INC, 1u, 1u);
     const uint32_t st = st_state(w);
@@ -230,7 +234,7 @@ __device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uin
       base -= kWave;
       continue;
     }
-    if (should_quit(a, deadline)) return false;
+    if (spin++ % kQuitPoll == 0 && should_quit(a, deadline)) return false;
     __builtin_amdgcn_s_sleep(1);
   }
 }
