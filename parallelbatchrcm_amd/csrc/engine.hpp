@@ -1089,8 +1089,10 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
           r = ld_sync(&a.brange[b]);
           if (r) break;
         }
-        if (ld_sync(&a.ctl->done)) { quit = 1; break; }
-        if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
+        if ((spin + 1) % kQuitPoll == 0) {  // the done word shares a line with the ticket: poll it rarely
+          if (ld_sync(&a.ctl->done)) { quit = 1; break; }
+          if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
+        }
         __builtin_amdgcn_s_sleep(1);
       }
       sm.quit = quit;

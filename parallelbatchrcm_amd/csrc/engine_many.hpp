@@ -133,9 +133,11 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
           }
           return;
         }
-        if (wave_uniform(ld_sync(&J.ctl->done))) break;  // no batch t: the job has finished
-        if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) break;
-        if (wave_uniform(ld_sync(&m.g->abort))) break;
+        if ((spin + 1) % kQuitPoll == 0) {  // the done word shares a line with the job's ticket: poll it rarely
+          if (wave_uniform(ld_sync(&J.ctl->done))) break;  // no batch t: the job has finished
+          if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) break;
+          if (wave_uniform(ld_sync(&m.g->abort))) break;
+        }
         __builtin_amdgcn_s_sleep(1);
       }
       if (lane == 0) atomic_add_u32(&m.helpers[cur], 0xFFFFFFFFu);  // leave the job
