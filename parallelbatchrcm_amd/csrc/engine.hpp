@@ -47,15 +47,20 @@ __device__ __forceinline__ uint32_t st_b(uint64_t w) { return (uint32_t)(w & 0x3
 
 enum : uint32_t { ERR_NONE = 0, ERR_TIMEOUT = 1, ERR_CAPACITY = 2, ERR_LONGROW = 3 };
 
-struct Control {
+// The ticket (an atomic per batch), the progress counter (an atomic per batch) and the done word (polled by idle
+// workers) sit on separate 128-byte lines, so the atomics and polls on one do not queue behind the others.
+struct alignas(128) Control {
   uint32_t ticket;    // next batch id handed to a worker
+  uint32_t pad0[31];
+  uint64_t progress;  // low 32: queue entries written; high 32: total to be written (0 = not yet known)
+  uint32_t pad1[30];
   uint32_t done;      // set once when the run is complete or aborted
   uint32_t error;     // ERR_*
   uint32_t nbatches;  // batch records used by the finished run
-  uint64_t progress;  // low 32: queue entries written; high 32: total to be written (0 = not yet known)
   uint32_t placed;    // nodes placed by the finished run
   uint32_t diag[9];   // first error: site, batch, begin, end, slots, offset, count, first child id, children
 };
+static_assert(offsetof(Control, progress) == 128 && offsetof(Control, done) == 256, "one line each");
 
 struct RunArgs {
   const int32_t* row_ptr;
