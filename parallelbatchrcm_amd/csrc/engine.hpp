@@ -182,11 +182,18 @@ __device__ __forceinline__ bool should_quit(const RunArgs& a, uint64_t deadline)
 // ---- look-back over predecessor status words (one wave) -----------------------------------------------
 // Claim completeness of every batch < b, without waiting. Lane l inspects batch (base - l); a position
 // below 0 is the virtual predecessor of batch 0, which counts as complete.
+// A look-back reads kLookWin status words per round trip (the other lanes stand in as neutral words), which
+// keeps the polls of waiting batches on fewer of the lines their predecessors are writing. Measured on MI355X
+// (3D 7-point 128^3, R-MAT 20): 16 words beat 64 by 3% and 8 by 1-3%.
+constexpr int kLookWin = 16;
+
 __device__ __forceinline__ bool claims_complete_before(const RunArgs& a, int64_t b) {
   int64_t base = b - 1;
   for (;;) {
     const int64_t idx = base - lane_id();
-    const uint32_t st = idx >= 0 ? st_state(ld_sync(&a.bstat[idx])) : (uint32_t)ST_DINC;
+    const uint32_t st = lane_id() >= kLookWin ? (uint32_t)ST_DLOC
+                        : idx >= 0            ? st_state(ld_sync(&a.bstat[idx]))
+                                              : (uint32_t)ST_DINC;
     const uint64_t incm = ballot(st >= ST_DINC);
     const uint64_t notm = ballot(st < ST_DLOC);
     if (incm) {
@@ -194,7 +201,7 @@ __device__ __forceinline__ bool claims_complete_before(const RunArgs& a, int64_t
       return (notm & (first - 1)) == 0;
     }
     if (notm) return false;
-    base -= kWave;
+    base -= kLookWin;
   }
 }
 
@@ -219,7 +226,9 @@ __device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uin
   uint32_t acc_off = 0, acc_nb = 0;
   for (uint32_t spin = 1;;) {
     const int64_t idx = base - lane_id();
-    const uint64_t w = idx >= 0 ? ld_sync(&a.bstat[idx]) : mk_status(ST_INC, 1u, 1u);
+    const uint64_t w = lane_id() >= kLookWin ? mk_status(ST_AGG, 0u, 0u)
+                       : idx >= 0            ? ld_sync(&a.bstat[idx])
+                                             : mk_status(ST_INC, 1u, 1u);
     const uint32_t st = st_state(w);
     const uint64_t incm = ballot(st == ST_INC);
     const uint64_t notm = ballot(st < ST_AGG);
@@ -236,7 +245,7 @@ __device__ __forceinline__ bool lookback_prefix(const RunArgs& a, int64_t b, uin
     } else if (notm == 0) {
       acc_off += wave_sum(st_a(w));
       acc_nb += wave_sum(st_b(w));
-      base -= kWave;
+      base -= kLookWin;
       continue;
     }
     if (spin++ % kQuitPoll == 0 && should_quit(a, deadline)) return false;
