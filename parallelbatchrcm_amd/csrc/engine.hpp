@@ -1103,7 +1103,7 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
           r = ld_sync(&a.brange[b]);
           if (r) break;
         }
-        if ((spin + 1) % kQuitPoll == 0) {  // the done word shares a line with the ticket: poll it rarely
+        if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done word rarely (measured faster)
           if (ld_sync(&a.ctl->done)) { quit = 1; break; }
           if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
         }

@@ -133,7 +133,7 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
           }
           return;
         }
-        if ((spin + 1) % kQuitPoll == 0) {  // the done word shares a line with the job's ticket: poll it rarely
+        if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done and abort words rarely
           if (wave_uniform(ld_sync(&J.ctl->done))) break;  // no batch t: the job has finished
           if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) break;
           if (wave_uniform(ld_sync(&m.g->abort))) break;
