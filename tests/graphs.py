@@ -106,3 +106,32 @@ def hub_graph(leaves, extra, seed, hubs=1, diag=True):
     perm = rng.permutation(n)
     e = [(int(perm[a]), int(perm[b])) for a, b in e]
     return from_edges(n, e, diag)
+
+
+def directed_with_empty_rows(n, extra, max_row, seed):
+    """A directed graph (rows are out-neighbour lists, not symmetric) in which every node is reachable from node 0
+    through a random tree, and the tree's leaves keep empty rows unless an extra edge lands on them: batches then
+    hold parents with no adjacency entries between parents with long rows (up to max_row entries)."""
+    rng = np.random.default_rng(seed)
+    rows = [[] for _ in range(n)]
+    order = rng.permutation(np.arange(1, n))
+    placed = [0]
+    for v in order:  # attach each node below a random placed node that still has room
+        while True:
+            p = placed[int(rng.integers(len(placed)))]
+            if len(rows[p]) < max_row:
+                break
+        rows[p].append(int(v))
+        placed.append(int(v))
+    inner = [i for i in range(n) if rows[i]]
+    heavy = inner[: max(1, len(inner) // 20)]  # extra edges go to a few nodes, whose rows grow to max_row
+    for _ in range(extra):
+        a = heavy[int(rng.integers(len(heavy)))]
+        if len(rows[a]) < max_row:
+            rows[a].append(int(rng.integers(n)))
+    rp = np.zeros(n + 1, np.int32)
+    cols = []
+    for i in range(n):
+        cols.extend(sorted(rows[i]))
+        rp[i + 1] = rp[i] + len(rows[i])
+    return rp, np.array(cols, np.int32)

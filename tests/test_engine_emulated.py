@@ -208,3 +208,25 @@ def test_one_workgroup_engine_matches_oracle():
                 else:
                     np.testing.assert_array_equal(perms[k], O.cm(rp, ci, starts[k]))
                 assert info[2 * k] == rp.shape[0] - 1
+
+
+DIRECTED = {"directed3000": (3000, 3000, 40, 1), "directed2000": (2000, 1500, 24, 2)}
+
+
+@pytest.mark.parametrize("name", list(DIRECTED))
+def test_directed_graph_with_empty_rows(name):
+    """Rows are out-neighbour lists (not symmetric) and half the nodes have empty rows, so batches hold parents
+    with no adjacency entries among parents with rows of up to 40 (both sort paths): the parent marks, the
+    chunked compaction and the cut against the serial algorithm, CM and plain BFS, one and many matrices."""
+    rp, ci = G.directed_with_empty_rows(*DIRECTED[name])
+    assert (np.diff(rp) == 0).sum() > rp.shape[0] // 3
+    ref = O.cm(rp, ci, 0)
+    for sort_mode in (0, 1, 2):
+        perm, _, info = emu_lib.run(rp, ci, 0, workers=3, sort_mode=sort_mode, engine="block")
+        assert info[0] == 0 and np.array_equal(perm, ref), (name, sort_mode)
+    want_order, want_level = O.bfs(rp, ci, 0)
+    perm, lev, info = emu_lib.run(rp, ci, 0, workers=3, bfs=True, engine="block")
+    assert info[0] == 0 and np.array_equal(perm, want_order) and np.array_equal(lev, want_level)
+    perms, _, info, done = emu_lib.run_many([(rp, ci), CASES["grid9x7"]], [0, 0], workers=3)
+    assert done == 2 and np.array_equal(perms[0], ref)
+    assert np.array_equal(perms[1], O.cm(*CASES["grid9x7"], 0))

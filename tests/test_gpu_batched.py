@@ -224,3 +224,18 @@ def test_batch_one_workgroup_engine_matches_oracle(wg_max_n):
         assert starts[k] == s, k
         assert np.array_equal(perms[k], want[::-1]), k
         assert bw[k] == wbw, k
+
+
+def test_batch_directed_graphs_with_empty_rows():
+    """Directed graphs with many empty rows beside symmetric ones, in one many-matrix launch, from node 0."""
+    mats = [P.CSR(*G.directed_with_empty_rows(3000, 3000, 40, 1)), P.CSR(*G.grid2d(7, 5)),
+            P.CSR(*G.directed_with_empty_rows(2000, 1500, 24, 2))]
+    bs = P.BatchSet.from_matrices(mats)
+    _say("batch directed", len(bs))
+    with P.RcmBatch(timeout_ms=20000) as b:
+        b.load(bs)
+        st = b.run(np.zeros(len(bs), np.int32), P.RCM_FLAG_STABLE)
+        assert (st == 0).all()
+        perms = b.perms()
+    for k, m in enumerate(mats):
+        assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, 0)), k

@@ -293,3 +293,20 @@ def test_level_synchronous_variant(ctx):
         assert ctx.run_status(0) == P.RCM_ENOTCONNECTED
     finally:
         ctx.set_param("variant", 0)
+
+
+@pytest.mark.parametrize("name,args", [("directed3000", (3000, 3000, 40, 1)), ("directed2000", (2000, 1500, 24, 2))])
+def test_directed_graph_with_empty_rows(ctx, name, args):
+    """Rows are out-neighbour lists and half the nodes have empty rows: batches mix parents without adjacency
+    entries and parents with up to 40, through both sort paths."""
+    rp, ci = G.directed_with_empty_rows(*args)
+    _say(name, rp.shape[0] - 1)
+    ctx.load(_csr(rp, ci))
+    ref = O.cm(rp, ci, 0)
+    try:
+        for sort in (0, 1, 2):
+            ctx.set_param("sort", sort)
+            ctx.run(0)
+            np.testing.assert_array_equal(ctx.perm(), ref, err_msg=f"{name} sort {sort}")
+    finally:
+        ctx.set_param("sort", 0)
