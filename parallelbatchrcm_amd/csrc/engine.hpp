@@ -765,22 +765,24 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   // waitcnt pass cannot count loads across the guarded blocks and waits for everything before each atomic).
   __builtin_amdgcn_s_waitcnt(0);
   if (a.claim_filter) {
-    // read the claim words (and row bounds) first; atomics only where the word is still larger than our code
+    // read the claim words first; atomics only where the word is still larger than our code, and the row bounds
+    // (the child's degree) only for those slots, in the same round trip as their atomics: a slot whose pre-read
+    // already lost can never own its child
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       oldv[k] = 0;
       crb[k] = 0;
       cre[k] = 0;
-      if ((uint32_t)(tid + k * T) < D) {
-        oldv[k] = claim_peek(a, cc[k]);
-        crb[k] = (uint32_t)a.row_ptr[cc[k]];
-        cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
-      }
+      if ((uint32_t)(tid + k * T) < D) oldv[k] = claim_peek(a, cc[k]);
     }
     __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
     for (int k = 0; k < EPT; ++k)
-      if ((uint32_t)(tid + k * T) < D && oldv[k] > code[k]) oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
+      if ((uint32_t)(tid + k * T) < D && oldv[k] > code[k]) {
+        oldv[k] = atomic_min_u64(&a.claim[cc[k]], code[k]);
+        crb[k] = (uint32_t)a.row_ptr[cc[k]];
+        cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
+      }
   } else {
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
