@@ -16,7 +16,7 @@
 //   Commit. Each batch publishes one 64-bit status word that moves through
 //       DISC_LOCAL (own claims done) -> DISC_INC (all claims up to here done)
 //       -> AGG (own child count, own child-batch count) -> INC (inclusive prefix of both).
-//   Successors read up to 64 predecessor words per wave load (decoupled look-back). A batch whose
+//   Successors read 16 predecessor words per wave load (decoupled look-back). A batch whose
 //   predecessors are all claim-complete before it claims ("early") owns every node its atomicMin won, after
 //   an in-LDS removal of duplicates within the batch; otherwise it re-reads its claim words once the
 //   look-back shows every predecessor has claimed.
