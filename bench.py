@@ -108,7 +108,7 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, a
             r = b.autotune(starts)
             out["autotune"] = {"variant": r.variant, "threads": r.threads, "workers": r.workers,
                                "candidates": r.candidates, "best_ms": r.best_ms, "default_ms": r.default_ms,
-                               "tuning_s": time.perf_counter() - t0}
+                               "helper_nodes": r.helper_nodes, "tuning_s": time.perf_counter() - t0}
         for _ in range(warmup):
             b.run(starts)
         walls, devs = [], []

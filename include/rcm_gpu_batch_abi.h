@@ -87,7 +87,11 @@ enum rcm_param {
   RCM_PARAM_WG_MAX_N = 10  /* rcm_batch only: matrices with at most this many rows and rows of at most 64 entries
                               are ordered by one workgroup each, level by level (no cross-workgroup chain);
                               the others by the batch engine. 0 = never (the default: that engine is not
-                              cleared for GPU use). */
+                              cleared for GPU use). */,
+  RCM_PARAM_HELPER_NODES = 11 /* rcm_batch only: a matrix of n rows is helped by at most n / value + 1 persistent
+                                 workgroups at once (at most 1024); default 8192. Fewer helpers per matrix leave
+                                 fewer workgroups waiting on narrow levels (throughput of large sets); more give
+                                 each matrix a shorter time to result. rcm_batch_autotune may change it. */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);
@@ -163,7 +167,8 @@ typedef struct rcm_batch_timing {
 int rcm_batch_create(int device, rcm_batch** out);
 int rcm_batch_destroy(rcm_batch* b);
 /* RCM_PARAM_WORKERS, RCM_PARAM_TIMEOUT_MS, RCM_PARAM_SORT, RCM_PARAM_GRAPH (1 = capture the reset and reorder
- * launches of rcm_batch_run in a hipGraph and replay it while the loaded set is unchanged; default 1). */
+ * launches of rcm_batch_run in a hipGraph and replay it while the loaded set is unchanged; default 1),
+ * RCM_PARAM_HELPER_NODES (applies to the loaded set at once). */
 int rcm_batch_set_param(rcm_batch* b, int param, int64_t value);
 /* Copy `count` matrices to the device (layout as rcm_gpu_batch_many) and validate each; status_out[k]
  * (nullable) is RCM_OK or RCM_EINVAL for matrix k. Returns RCM_EINVAL if any matrix is malformed. */
@@ -255,7 +260,8 @@ int rcm_selftest_primitives(int device, const uint32_t* in, uint32_t* out);
 
 /* ---- autotuner ---- */
 /* No reference counterpart. Times every candidate (engine variant x persistent workgroups, and for rcm_ctx the
- * level-synchronous variant 5) on the loaded work
+ * level-synchronous variant 5; rcm_batch then also tries helper caps of 8192, 32768 and 131072 nodes) on the
+ * loaded work
  * (`repeats` runs each, median of the device time) and keeps the fastest in the context; the order never
  * depends on the choice. rcm_ctx_autotune runs the matrix from `start`; rcm_batch_autotune runs the whole
  * loaded set from start[k]. */
@@ -266,6 +272,8 @@ typedef struct rcm_tune_result {
   int32_t threads;      /* block size of the chosen variant */
   double best_ms;       /* median device time of the choice */
   double default_ms;    /* median device time of variant 0 with automatic workers */
+  int32_t helper_nodes; /* rcm_batch_autotune: chosen RCM_PARAM_HELPER_NODES (0 from rcm_ctx_autotune) */
+  int32_t reserved;
 } rcm_tune_result;
 int rcm_ctx_autotune(rcm_ctx* ctx, int32_t start, int32_t repeats, rcm_tune_result* out);
 int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_tune_result* out);

@@ -68,6 +68,7 @@ PARAM_PROFILE = 7
 PARAM_GRAPH = 8
 PARAM_CLAIM_FILTER = 9
 PARAM_WG_MAX_N = 10
+PARAM_HELPER_NODES = 11
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -660,7 +661,8 @@ class RcmBatch:
         self._bs: Optional[BatchSet] = None
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH,
-               "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT, "wg_max_n": PARAM_WG_MAX_N}
+               "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT, "wg_max_n": PARAM_WG_MAX_N,
+               "helper_nodes": PARAM_HELPER_NODES}
 
     def autotune(self, starts, repeats: int = 1) -> TuneResultT:
         """Time every engine variant x workgroup count on the loaded set and keep the fastest."""

@@ -115,6 +115,8 @@ class TuneResultT(C.Structure):
         ("threads", C.c_int32),
         ("best_ms", C.c_double),
         ("default_ms", C.c_double),
+        ("helper_nodes", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

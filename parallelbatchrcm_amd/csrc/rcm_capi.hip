@@ -29,6 +29,14 @@ using namespace rcm;
 namespace {
 
 constexpr int kDefaultSegThreshold = 24;
+constexpr int64_t kDefaultHelperNodes = 8192;  // a matrix of n rows: at most n / 8192 + 1 helpers
+
+static_assert(sizeof(rcm_tune_result) == 40, "the Python mirror (TuneResultT) assumes this layout");
+
+// Workgroups that may help one matrix of n rows at once.
+static int32_t helper_cap(int64_t n, int64_t nodes_per_helper) {
+  return (int32_t)std::min<int64_t>(n / nodes_per_helper + 1, 1024);
+}
 constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56% fewer HBM writes
 // Many-matrix launches: matrices with at most this many rows, and rows of at most kWgMaxRow entries, are
 // ordered by one workgroup each (engine_wg.hpp); the others by the batch engine's helpers. Off (0): the first
@@ -336,6 +344,8 @@ struct rcm_batch {
   int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
   int64_t wg_max_n = kDefaultWgMaxN;
+  int64_t helper_nodes = kDefaultHelperNodes;  // RCM_PARAM_HELPER_NODES
+  std::vector<JobDesc> jobs_host;              // host copy of the loaded set's job descriptors
   int max_res_wg = 0;                // co-resident workgroups of the one-workgroup-per-matrix engine
   int64_t variant = 0;
   int max_resident = 0;              // of the current variant
@@ -1187,6 +1197,17 @@ int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
       batch_drop_graphs(b);  // captured graphs hold the old launch parameters
       b->max_resident = b->max_res[v];
       return RCM_OK;
+    case RCM_PARAM_HELPER_NODES: {
+      if (v < 1) return RCM_EINVAL;
+      b->helper_nodes = v;
+      batch_drop_graphs(b);
+      if (b->count == 0 || b->jobs_host.empty()) return RCM_OK;
+      for (size_t k = 0; k < b->jobs_host.size(); ++k) b->jobs_host[k].cap = helper_cap(b->jobs_host[k].n, v);
+      RCM_HIP(hipMemcpyAsync(b->jobs, b->jobs_host.data(), b->jobs_host.size() * sizeof(JobDesc),
+                             hipMemcpyHostToDevice, b->stream));
+      RCM_HIP(hipStreamSynchronize(b->stream));
+      return RCM_OK;
+    }
     default: return RCM_EINVAL;
   }
 }
@@ -1253,7 +1274,7 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
     J.brange = b->brange + pb;
     J.ctl = b->ctl + k;
     J.n = n[k];
-    J.cap = (int32_t)std::min<int64_t>((int64_t)n[k] / 8192 + 1, 1024);  // helpers: ~1 per 8192 nodes
+    J.cap = helper_cap(n[k], b->helper_nodes);
     po += n[k];
     pb += n[k] + 1;
   }
@@ -1267,6 +1288,7 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   if (ci_end > 0)
     RCM_HIP(hipMemcpyAsync(b->ci, col_idx_cat, (size_t)ci_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemcpyAsync(b->jobs, jobs.data(), C * sizeof(JobDesc), hipMemcpyHostToDevice, b->stream));
+  b->jobs_host = jobs;
   RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemcpyAsync(b->d_nnz, nnz, C * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
   std::vector<int32_t> blk;
@@ -1808,6 +1830,30 @@ int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_
   }
   rcm_batch_set_param(b, RCM_PARAM_VARIANT, best.variant);
   rcm_batch_set_param(b, RCM_PARAM_WORKERS, best.workers);
+  // then the helper cap, with the chosen engine
+  const int64_t keep_h = b->helper_nodes;
+  int64_t best_h = keep_h;
+  for (int64_t h : {kDefaultHelperNodes, 4 * kDefaultHelperNodes, 16 * kDefaultHelperNodes}) {
+    if (h == keep_h) continue;  // timed above
+    int rc = rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, h);
+    std::vector<double> t;
+    for (int r = 0; rc == RCM_OK && r < repeats; ++r) {
+      rc = rcm_batch_run(b, start, RCM_FLAG_STABLE, st.data());
+      if (rc == RCM_OK) t.push_back(b->timing.run_ms);
+    }
+    if (rc == RCM_EDEVICE || rc == RCM_ETIMEOUT || rc == RCM_EINVAL || rc == RCM_ENOTLOADED) {
+      rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, keep_h);
+      return rc;
+    }
+    ++count;
+    const double ms = median_of(t);
+    if (ms < best.best_ms) {
+      best.best_ms = ms;
+      best_h = h;
+    }
+  }
+  rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, best_h);
+  best.helper_nodes = (int32_t)best_h;
   best.candidates = count;
   best.threads = kVarThreads[best.variant];
   b->have_perm = false;

@@ -193,3 +193,11 @@ def test_server_create_without_gpu_fails_cleanly():
         _native.lib().rcm_server_destroy(h)
     else:
         assert rc == P.RCM_EDEVICE
+
+
+def test_tune_result_mirror_layout():
+    """rcm_tune_result is 40 bytes in the library (a static_assert there); the ctypes mirror must match."""
+    import ctypes
+
+    assert ctypes.sizeof(_native.TuneResultT) == 40
+    assert _native.TuneResultT.helper_nodes.offset == 32

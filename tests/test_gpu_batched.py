@@ -239,3 +239,24 @@ def test_batch_directed_graphs_with_empty_rows():
         perms = b.perms()
     for k, m in enumerate(mats):
         assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, 0)), k
+
+
+def test_batch_helper_cap_keeps_results():
+    """RCM_PARAM_HELPER_NODES changes how many workgroups help one matrix, never the orders."""
+    bs = P.workloads.batch_mix(24, n_lo=500, n_hi=20000, seed=13)
+    _say("batch helper caps", len(bs))
+    with P.RcmBatch(timeout_ms=30000) as b:
+        b.load(bs)
+        starts, _, st = b.find_start()
+        assert (st == 0).all()
+        got = {}
+        for h in (1, 8192, 1 << 30):
+            b.set_param("helper_nodes", h)
+            assert (b.run(starts, P.RCM_FLAG_STABLE) == 0).all()
+            got[h] = b.perms()
+        with pytest.raises(Exception):
+            b.set_param("helper_nodes", 0)
+    for k in range(len(bs)):
+        want = O.cm(bs.matrix(k).row_offsets, bs.matrix(k).col_ids, int(starts[k]))
+        for h, perms in got.items():
+            assert np.array_equal(perms[k], want), (k, h)
