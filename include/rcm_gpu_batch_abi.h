@@ -84,14 +84,17 @@ enum rcm_param {
   RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
   RCM_PARAM_CLAIM_FILTER = 9 /* 0 = atomicMin on every adjacency entry; 1 = read the claim word first and skip
                                 entries already claimed by a smaller code (plain load); 2 = same, device-scope */,
-  RCM_PARAM_WG_MAX_N = 10  /* rcm_batch only: matrices with at most this many rows and rows of at most 64 entries
-                              are ordered by one workgroup each, level by level (no cross-workgroup chain);
-                              the others by the batch engine. 0 = never (the default: that engine is not
-                              cleared for GPU use). */,
+  RCM_PARAM_WG_MAX_N = 10  /* rcm_batch only, reserved: 0 is accepted, any other value is RCM_EINVAL (a
+                              one-workgroup-per-matrix engine is not part of the library). */,
   RCM_PARAM_HELPER_NODES = 11 /* rcm_batch only: a matrix of n rows is helped by at most n / value + 1 persistent
                                  workgroups at once (at most 1024); default 8192. Fewer helpers per matrix leave
                                  fewer workgroups waiting on narrow levels (throughput of large sets); more give
-                                 each matrix a shorter time to result. rcm_batch_autotune may change it. */
+                                 each matrix a shorter time to result. rcm_batch_autotune may change it. */,
+  RCM_PARAM_BATCH_CAP = 12    /* rcm_ctx only: batch records a run may use; 0 (default) = n + 1, the most a run
+                                 can need. A smaller value makes a run that needs more records stop with
+                                 RCM_ECAPACITY (every workgroup exits; the context stays usable) -- the error
+                                 path of the reference's oversize-batch __trap (include/device/batchWorker.cuh:
+                                 539-543), which cannot be reached otherwise. */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);

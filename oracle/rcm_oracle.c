@@ -312,36 +312,88 @@ static int cmp_i32(const void* a, const void* b) {
   return (x > y) - (x < y);
 }
 
+/* (column, input position) of one entry: sorting by both ranks equal columns by input position. */
+typedef struct {
+  int32_t col;
+  int64_t pos;
+} ColPos;
+static int cmp_colpos(const void* a, const void* b) {
+  const ColPos* x = (const ColPos*)a;
+  const ColPos* y = (const ColPos*)b;
+  if (x->col != y->col) return x->col < y->col ? -1 : 1;
+  return (x->pos > y->pos) - (x->pos < y->pos);
+}
+
 /* convert (src/CSR.cpp:170-208): sort entries by (row, col), duplicates kept. Entries are given as parallel
- * row/col arrays; the column structure of the result does not depend on std::sort's tie order. */
-static int build_sorted_csr(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int32_t** rp_out,
-                            int32_t** ci_out) {
+ * row/col arrays; the column structure of the result does not depend on std::sort's tie order. With values
+ * (vals != NULL), duplicates keep their input order and *vals_out receives the values in the same order. */
+static int build_sorted_csr(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, const double* vals,
+                            int32_t** rp_out, int32_t** ci_out, double** vals_out) {
   int32_t* rp = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
   int32_t* ci = (int32_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int32_t));
-  int32_t* fill = (int32_t*)malloc((size_t)n * sizeof(int32_t));
-  if (!rp || !ci || !fill) { free(rp); free(ci); free(fill); return -1; }
+  int64_t* src = (int64_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int64_t));
+  int32_t* fill = (int32_t*)malloc(((size_t)n + 1) * sizeof(int32_t));
+  double* vo = vals ? (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double)) : NULL;
+  if (!rp || !ci || !src || !fill || (vals && !vo)) { free(rp); free(ci); free(src); free(fill); free(vo); return -1; }
   for (int64_t k = 0; k < m; ++k) rp[rows[k] + 1]++;
   for (int32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
   for (int32_t i = 0; i < n; ++i) fill[i] = rp[i];
-  for (int64_t k = 0; k < m; ++k) ci[fill[rows[k]]++] = cols[k];
-  for (int32_t i = 0; i < n; ++i)
-    if (rp[i + 1] - rp[i] > 1) qsort(ci + rp[i], (size_t)(rp[i + 1] - rp[i]), sizeof(int32_t), cmp_i32);
+  for (int64_t k = 0; k < m; ++k) {
+    const int32_t at = fill[rows[k]]++;
+    ci[at] = cols[k];
+    src[at] = k; /* row buckets keep input order */
+  }
+  if (!vals) {
+    for (int32_t i = 0; i < n; ++i)
+      if (rp[i + 1] - rp[i] > 1) qsort(ci + rp[i], (size_t)(rp[i + 1] - rp[i]), sizeof(int32_t), cmp_i32);
+  } else {
+    int32_t longest = 0;
+    for (int32_t i = 0; i < n; ++i) longest = rp[i + 1] - rp[i] > longest ? rp[i + 1] - rp[i] : longest;
+    ColPos* tmp = (ColPos*)malloc((size_t)(longest > 0 ? longest : 1) * sizeof(ColPos));
+    if (!tmp) { free(rp); free(ci); free(src); free(fill); free(vo); return -1; }
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t b = rp[i], d = rp[i + 1] - rp[i];
+      for (int32_t t = 0; t < d; ++t) { tmp[t].col = ci[b + t]; tmp[t].pos = src[b + t]; }
+      qsort(tmp, (size_t)d, sizeof(ColPos), cmp_colpos);
+      for (int32_t t = 0; t < d; ++t) { ci[b + t] = tmp[t].col; vo[b + t] = vals[tmp[t].pos]; }
+    }
+    free(tmp);
+  }
+  free(src);
   free(fill);
   *rp_out = rp;
   *ci_out = ci;
+  if (vals_out) *vals_out = vo;
   return 0;
+}
+
+int64_t orc_convert(int32_t rows, int64_t m, const int32_t* row_ids, const int32_t* col_ids, const double* vals,
+                    int32_t** rp_out, int32_t** ci_out, double** vals_out) {
+  if (rows < 1 || m < 0 || !row_ids || !col_ids || !rp_out || !ci_out) return -1;
+  for (int64_t k = 0; k < m; ++k)
+    if (row_ids[k] < 0 || row_ids[k] >= rows) return -1;
+  if (build_sorted_csr(rows, m, row_ids, col_ids, vals, rp_out, ci_out, vals ? vals_out : NULL)) return -1;
+  return m;
 }
 
 int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* ci0, int32_t check_sym,
                     int32_t** rp_out, int32_t** ci_out, int64_t* nnz_out) {
-  if (!valid_csr(n, rp0, ci0) || rp0[n] != nnz || !rp_out || !ci_out || !nnz_out) return -1;
+  return orc_prepare_values(n, nnz, rp0, ci0, NULL, check_sym, rp_out, ci_out, NULL, nnz_out);
+}
+
+int64_t orc_prepare_values(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* ci0, const double* v0,
+                           int32_t check_sym, int32_t** rp_out, int32_t** ci_out, double** vals_out,
+                           int64_t* nnz_out) {
+  if (!valid_csr(n, rp0, ci0) || rp0[n] != nnz || !rp_out || !ci_out || !nnz_out || (v0 && !vals_out)) return -1;
   for (int64_t k = 0; k < nnz; ++k)
     if (ci0[k] < 0 || ci0[k] >= n) return -1;
   int32_t* rp = (int32_t*)malloc(((size_t)n + 1) * sizeof(int32_t));
   int32_t* ci = (int32_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int32_t));
+  double* va = v0 ? (double*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double)) : NULL;
   int64_t m = nnz;
   memcpy(rp, rp0, ((size_t)n + 1) * sizeof(int32_t));
   memcpy(ci, ci0, (size_t)nnz * sizeof(int32_t));
+  if (va) memcpy(va, v0, (size_t)nnz * sizeof(double));
 
   /* (1) symmetrise (include/Graph.h:77-157): pairs with an odd number of stored off-diagonal occurrences
    * stay in `entries` after the toggle pass; every stored occurrence of such a pair gets its transpose. */
@@ -369,29 +421,37 @@ int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* c
       int64_t cap = m * 2 + 1;
       int32_t* rows = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
       int32_t* cols = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
+      double* vals = va ? (double*)malloc((size_t)cap * sizeof(double)) : NULL;
       int64_t t = 0;
       for (int32_t r = 0; r < n; ++r)
         for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
           int32_t c = ci[k];
           rows[t] = r;
           cols[t] = c;
+          if (vals) vals[t] = va[k];
           ++t;
           if (r == c) continue;
           uint32_t lo = (uint32_t)r < (uint32_t)c ? (uint32_t)r : (uint32_t)c;
           uint32_t hi = (uint32_t)r < (uint32_t)c ? (uint32_t)c : (uint32_t)r;
           uint64_t key = ((uint64_t)lo << 32) | hi;
           if (bsearch(&key, odd, (size_t)nodd, sizeof(uint64_t), cmp_u64)) {
-            rows[t] = c;
+            rows[t] = c; /* the transpose right after the entry, with its value (Graph.h:140-146) */
             cols[t] = r;
+            if (vals) vals[t] = va[k];
             ++t;
           }
         }
       free(rp);
       free(ci);
-      if (build_sorted_csr(n, t, rows, cols, &rp, &ci)) { free(rows); free(cols); free(odd); return -1; }
+      free(va);
+      va = NULL;
+      if (build_sorted_csr(n, t, rows, cols, vals, &rp, &ci, vals ? &va : NULL)) {
+        free(rows); free(cols); free(vals); free(odd); return -1;
+      }
       m = t;
       free(rows);
       free(cols);
+      free(vals);
     }
     free(odd);
   }
@@ -407,7 +467,7 @@ int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* c
   for (int32_t r = 0; r < n; ++r)
     if (rp[r + 1] != rp[r]) { minn = r; break; }
   if (minn == n) { /* every row empty: the reference indexes visited[rows] */
-    free(visited); free(queue); free(rp); free(ci);
+    free(visited); free(queue); free(rp); free(ci); free(va);
     return -3;
   }
   int64_t head = 0, tail = 0;
@@ -433,7 +493,7 @@ int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* c
     int32_t i = cursor;
     if (i == last_i) {
       /* the previous connection did not make i visited: the reference re-adds it forever */
-      free(visited); free(queue); free(con_a); free(con_b); free(rp); free(ci);
+      free(visited); free(queue); free(con_a); free(con_b); free(rp); free(ci); free(va);
       return -3;
     }
     int32_t a, b;
@@ -460,24 +520,36 @@ int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* rp0, const int32_t* c
     int64_t t = m + 2 * added;
     int32_t* rows = (int32_t*)malloc((size_t)t * sizeof(int32_t));
     int32_t* cols = (int32_t*)malloc((size_t)t * sizeof(int32_t));
+    double* vals = va ? (double*)malloc((size_t)t * sizeof(double)) : NULL;
     int64_t w = 0;
     for (int32_t r = 0; r < n; ++r)
-      for (int64_t k = rp[r]; k < rp[r + 1]; ++k) { rows[w] = r; cols[w] = ci[k]; ++w; }
-    for (int64_t q = 0; q < added; ++q) {
-      rows[w] = con_b[q]; cols[w] = con_a[q]; ++w; /* (second, first) then (first, second), :240-249 */
-      rows[w] = con_a[q]; cols[w] = con_b[q]; ++w;
+      for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+        rows[w] = r; cols[w] = ci[k];
+        if (vals) vals[w] = va[k];
+        ++w;
+      }
+    for (int64_t q = 0; q < added; ++q) { /* value 1 in both directions (Graph.h:240-249) */
+      rows[w] = con_b[q]; cols[w] = con_a[q]; if (vals) vals[w] = 1.0; ++w; /* (second, first), (first, second) */
+      rows[w] = con_a[q]; cols[w] = con_b[q]; if (vals) vals[w] = 1.0; ++w;
     }
     free(rp);
     free(ci);
-    if (build_sorted_csr(n, t, rows, cols, &rp, &ci)) { free(rows); free(cols); free(con_a); free(con_b); return -1; }
+    free(va);
+    va = NULL;
+    if (build_sorted_csr(n, t, rows, cols, vals, &rp, &ci, vals ? &va : NULL)) {
+      free(rows); free(cols); free(vals); free(con_a); free(con_b); return -1;
+    }
     m = t;
     free(rows);
     free(cols);
+    free(vals);
   }
   free(con_a);
   free(con_b);
   *rp_out = rp;
   *ci_out = ci;
+  if (vals_out) *vals_out = va;
+  else free(va);
   *nnz_out = m;
   return added;
 }

@@ -66,6 +66,16 @@ int64_t orc_verify_order(int32_t n, const int32_t* row_ptr, const int32_t* col_i
  * entries), -1 on invalid arguments. */
 int64_t orc_prepare(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int32_t check_sym,
                     int32_t** row_ptr_out, int32_t** col_idx_out, int64_t* nnz_out);
+/* The same with values (Graph<double>::prepareRCM): an added transpose carries its entry's value, a connection
+ * carries 1 in both directions, and the rebuilt CSR keeps duplicates in input order. vals may be NULL (then
+ * *vals_out is left alone). */
+int64_t orc_prepare_values(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                           int32_t check_sym, int32_t** row_ptr_out, int32_t** col_idx_out, double** vals_out,
+                           int64_t* nnz_out);
+/* convert(CSR, COO) (src/CSR.cpp:170-208) with values: the rows x ? COO (row_ids, col_ids, vals: m entries; vals
+ * nullable) as a CSR sorted by (row, col), duplicates in input order. Returns m, or -1 on a row out of range. */
+int64_t orc_convert(int32_t rows, int64_t m, const int32_t* row_ids, const int32_t* col_ids, const double* vals,
+                    int32_t** row_ptr_out, int32_t** col_idx_out, double** vals_out);
 
 void orc_free(void* p);
 

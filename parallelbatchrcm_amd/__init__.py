@@ -69,6 +69,7 @@ PARAM_GRAPH = 8
 PARAM_CLAIM_FILTER = 9
 PARAM_WG_MAX_N = 10
 PARAM_HELPER_NODES = 11
+PARAM_BATCH_CAP = 12
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -534,7 +535,7 @@ class RcmContext:
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT,
                "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT,
-               "profile": PARAM_PROFILE, "claim_filter": PARAM_CLAIM_FILTER}
+               "profile": PARAM_PROFILE, "claim_filter": PARAM_CLAIM_FILTER, "batch_cap": PARAM_BATCH_CAP}
 
     def set_param(self, name: str, value: int) -> None:
         _check(self._lib.rcm_ctx_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
@@ -698,6 +699,12 @@ class RcmBatch:
         if rc in (RCM_EDEVICE, RCM_ETIMEOUT, RCM_ENOTLOADED, RCM_EINVAL):
             raise RcmError(rc, "rcm_batch_run")
         return st
+
+    def run_status(self, starts, flags: int = RCM_FLAG_STABLE):
+        """Reorder every matrix; returns (return code, per-matrix status) without raising."""
+        st = np.zeros(len(self._bs), np.int32)
+        rc = _native.lib().rcm_batch_run(self._h, _p32(_i32(starts)), flags, _p32(st))
+        return int(rc), st
 
     def bandwidth(self) -> np.ndarray:
         bw = np.zeros(len(self._bs), np.int64)

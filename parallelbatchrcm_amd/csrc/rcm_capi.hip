@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -19,7 +20,6 @@
 #include <vector>
 
 #include "engine_many.hpp"
-#include "engine_wg.hpp"
 #include "hip_exec.hpp"
 #include "level_cm.hpp"
 #include "rcm_gpu_batch_abi.h"
@@ -41,11 +41,6 @@ static int32_t helper_cap(int64_t n, int64_t nodes_per_helper) {
   return (int32_t)std::min<int64_t>(n / nodes_per_helper + 1, 1024);
 }
 constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56% fewer HBM writes
-// Many-matrix launches: matrices with at most this many rows, and rows of at most kWgMaxRow entries, are
-// ordered by one workgroup each (engine_wg.hpp); the others by the batch engine's helpers. Off (0): the first
-// GPU run of that engine ended in a memory fault whose cause has not been shown; the engine's device code is
-// exercised only by the CPU emulator tests.
-constexpr int64_t kDefaultWgMaxN = 0;
 
 // Engine variants (RCM_PARAM_VARIANT): block size T, batch width (adjacency slots S, parents P), radix digit.
 // The order never depends on the variant (batch boundaries do not change it); the speed does.
@@ -151,6 +146,7 @@ struct rcm_ctx {
   int64_t variant = 0;
   int64_t strict = 0;
   int64_t profile = 0;
+  int64_t batch_cap = 0;     // RCM_PARAM_BATCH_CAP: batch records a run may use (0 = n + 1, all it can need)
   uint64_t* prof = nullptr;  // kProfSlots words
   // matrix
   int32_t n = 0;
@@ -268,7 +264,7 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   a.level = bfs ? c->level : nullptr;
   a.ctl = c->ctl;
   a.n = c->n;
-  a.bcap = c->n + 1;
+  a.bcap = (c->batch_cap > 0 && c->batch_cap < (int64_t)c->n + 1) ? (int32_t)c->batch_cap : c->n + 1;
   a.budget_ticks = (uint64_t)c->timeout_ms * 100000ull;
   a.sort_mode = (int32_t)c->sort_mode;
   a.claim_filter = (int32_t)c->claim_filter;
@@ -346,10 +342,8 @@ struct rcm_batch {
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
-  int64_t wg_max_n = kDefaultWgMaxN;
   int64_t helper_nodes = kDefaultHelperNodes;  // RCM_PARAM_HELPER_NODES
   std::vector<JobDesc> jobs_host;              // host copy of the loaded set's job descriptors
-  int max_res_wg = 0;                // co-resident workgroups of the one-workgroup-per-matrix engine
   int64_t variant = 0;
   int max_resident = 0;              // of the current variant
   int max_res[kVariants] = {};       // co-resident workgroups of each variant
@@ -377,6 +371,7 @@ struct rcm_batch {
   int32_t nblk_order = 0;
   ManyControl* g = nullptr;
   // pinned host mirrors
+  ManyControl* h_g = nullptr;  // the launch control of the last launch (its abort word: the watchdog fired)
   Control* h_ctl = nullptr;
   int32_t* h_tail = nullptr;
   // graph of the run launches (reset + reorder [+ reverse]), per reverse flag
@@ -387,8 +382,6 @@ struct rcm_batch {
   bool reversed = false;
   std::vector<int32_t> ok_jobs;  // matrices whose last run completed
   std::vector<uint32_t> maxrow;  // longest row per matrix (from the load's validation)
-  std::vector<uint8_t> wg;       // per matrix: ordered by one workgroup
-  int32_t n_helper = 0;          // `order` lists the helper engine's matrices first, then the others
   std::vector<int32_t> blk_order_h;  // host copy of the block map of `order`
   rcm_batch_timing_t timing{};
 };
@@ -471,14 +464,15 @@ void batch_release(rcm_batch* b) {
   b->hcap = 0;
 }
 
-// Device arena of at least `bytes` (grown unless fixed) and pinned mirrors for `count` matrices.
-int batch_reserve(rcm_batch* b, size_t bytes, int32_t count) {
+// Device arena of at least `bytes` (grown unless fixed; a grown arena gets 25% headroom, an `exact` one none)
+// and pinned mirrors for `count` matrices.
+int batch_reserve(rcm_batch* b, size_t bytes, int32_t count, bool exact = false) {
   if (bytes > b->arena_cap) {
     if (b->arena_fixed) return RCM_ENOMEM;
     if (b->arena) (void)hipFree(b->arena);
     b->arena = nullptr;
     b->arena_cap = 0;
-    const size_t want = bytes + bytes / 4;
+    const size_t want = exact ? bytes : bytes + bytes / 4;
     if (hipMalloc(reinterpret_cast<void**>(&b->arena), want) != hipSuccess) return RCM_ENOMEM;
     b->arena_cap = want;
   }
@@ -519,69 +513,31 @@ int batch_workers(rcm_batch* b) {
   return std::max(1, std::min(w, b->max_resident));
 }
 
-// One pass over a list of matrices: d_list[0, nhelper) by the batch engine's helpers (d_blk's first nhelper + 1
-// entries are their block map, nblk_helper blocks), then d_list[nhelper, nhelper + nwg) with one workgroup each.
-void enqueue_many(rcm_batch* b, const int32_t* d_list, const int32_t* d_blk, int32_t nblk_helper, int32_t nhelper,
-                  int32_t nwg, bool bfs) {
-  if (nhelper > 0) {
-    const ManyArgs m = batch_args(b, d_list, nhelper);
-    hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk_helper), dim3(256), 0, b->stream, m, d_blk);
-    void* kern = many_kernel((int)b->variant, bfs);
-    ManyArgs mm = m;
-    void* args[] = {&mm};
-    (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
-  }
-  if (nwg > 0) {
-    const ManyArgs m = batch_args(b, d_list + nhelper, nwg);
-    hipLaunchKernelGGL(k_wg_prepare, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0, b->stream, m);
-    const unsigned grid = (unsigned)std::min(nwg, b->max_res_wg);
-    if (bfs)
-      hipLaunchKernelGGL(k_order_wg<true>, dim3(grid), dim3(kWgT), 0, b->stream, m);
-    else
-      hipLaunchKernelGGL(k_order_wg<false>, dim3(grid), dim3(kWgT), 0, b->stream, m);
-  }
+// One pass over a list of matrices (d_list, count entries; d_blk is their block map of nblk blocks): the reset
+// and the persistent many-matrix launch.
+void enqueue_many(rcm_batch* b, const int32_t* d_list, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs) {
+  if (count <= 0) return;
+  const ManyArgs m = batch_args(b, d_list, count);
+  hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, m, d_blk);
+  void* kern = many_kernel((int)b->variant, bfs);
+  ManyArgs mm = m;
+  void* args[] = {&mm};
+  (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
 }
 
-bool wg_eligible(const rcm_batch* b, int32_t k) {
-  return b->wg_max_n > 0 && (int64_t)b->n[k] <= b->wg_max_n && b->maxrow[k] <= (uint32_t)kWgMaxRow;
-}
-
-// `order` = the helper engine's matrices, then the one-workgroup engine's, each largest first; uploads it and
-// its block map.
-int batch_partition(rcm_batch* b) {
-  const int32_t C = b->count;
-  std::vector<int32_t> by_size((size_t)C);
-  for (int32_t k = 0; k < C; ++k) by_size[k] = k;
-  std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t x, int32_t y) { return b->nnz[x] > b->nnz[y]; });
-  b->wg.assign((size_t)C, 0);
-  b->order.clear();
-  for (int32_t k : by_size)
-    if (!wg_eligible(b, k)) b->order.push_back(k);
-  b->n_helper = (int32_t)b->order.size();
-  for (int32_t k : by_size)
-    if (wg_eligible(b, k)) {
-      b->order.push_back(k);
-      b->wg[k] = 1;
-    }
-  b->nblk_order = build_blk(b, b->order.data(), C, b->blk_order_h);
-  RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-  RCM_HIP(hipMemcpyAsync(b->blk_order, b->blk_order_h.data(), ((size_t)C + 1) * sizeof(int32_t),
-                         hipMemcpyHostToDevice, b->stream));
-  RCM_HIP(hipStreamSynchronize(b->stream));
-  return RCM_OK;
-}
-
-// Per-matrix status from the control blocks of the last launch (jobs listed in `order`).
+// Per-matrix status from the control blocks of the last launch (jobs listed in `order`). A matrix the launch
+// left unfinished because its watchdog fired (the launch's abort word) is RCM_ETIMEOUT.
 int collect_status(rcm_batch* b, const int32_t* order, int32_t count, int32_t* status, int64_t* batches) {
   int worst = RCM_OK;
   int64_t nb = 0;
+  const bool aborted = b->h_g->abort != 0;
   for (int32_t i = 0; i < count; ++i) {
     const int32_t k = order[i];
     const Control& h = b->h_ctl[k];
     int r = RCM_OK;
     if (h.error == ERR_TIMEOUT) r = RCM_ETIMEOUT;
     else if (h.error == ERR_CAPACITY || h.error == ERR_LONGROW) r = RCM_ECAPACITY;
-    else if (h.error != ERR_NONE || h.nbatches == 0) r = RCM_EDEVICE;
+    else if (h.error != ERR_NONE || h.nbatches == 0) r = aborted ? RCM_ETIMEOUT : RCM_EDEVICE;
     else if ((int32_t)h.placed != b->n[k]) r = RCM_ENOTCONNECTED;
     nb += h.nbatches;
     if (status) status[k] = r;
@@ -632,13 +588,20 @@ struct ServerJob {
   std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
 };
 
-// Arena bytes of one matrix inside a lane's set (batch_layout without the alignment of shared buffers).
-size_t job_bytes(int32_t n, int64_t nnz) {
-  const size_t N = (size_t)n;
-  return 4 * (N + 1) + 4 * (size_t)nnz + 12 * N + 16 * N + 16 * (N + 1) + sizeof(Control) + sizeof(JobDesc) + 4 * 3 +
-         4 * 96 + 8 + 12;
-}
-constexpr size_t kLaneSlack = 18 * 256 + 256;  // alignment of the 18 shared buffers of a lane's layout
+// Admission counts what rcm_batch_load will lay out: the arena bytes of a group are batch_layout's total over
+// the group's summed row offsets, columns, rows and matrices.
+struct GroupSize {
+  size_t rp = 0, ci = 0, n = 0, count = 0;
+  GroupSize plus(int32_t jn, int64_t jnnz) const {
+    GroupSize g = *this;
+    g.rp += (size_t)jn + 1;
+    g.ci += (size_t)jnnz;
+    g.n += (size_t)jn;
+    g.count += 1;
+    return g;
+  }
+  size_t bytes() const { return batch_layout(rp, ci, n, count).total; }
+};
 
 }  // namespace
 
@@ -744,12 +707,13 @@ void lane_loop(rcm_server* s, int lane) {
       s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
       if (s->pending.empty()) return;  // stopping, nothing left
       // a group: first come first served, looking at most 8 requests ahead for ones that fit the lane
-      size_t used = kLaneSlack;
+      GroupSize gs;
       size_t skipped = 0;
       for (size_t i = 0; i < s->pending.size() && skipped < 8 && (int32_t)grp.size() < s->cfg.slots;) {
         const auto& jp = s->pending[i];
-        if (used + jp->bytes <= s->lane_bytes) {
-          used += jp->bytes;
+        const GroupSize next = gs.plus(jp->n, jp->nnz);
+        if (next.bytes() <= s->lane_bytes) {
+          gs = next;
           grp.push_back(jp);
           s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
         } else {
@@ -766,6 +730,7 @@ void lane_loop(rcm_server* s, int lane) {
       s->st.inflight += (int32_t)grp.size();
       s->st.max_inflight = std::max(s->st.max_inflight, s->st.inflight);
       s->st.pending = (int32_t)s->pending.size();
+      const size_t used = grp.empty() ? 0 : gs.bytes();
       s->inflight_bytes += (int64_t)used;
       s->st.arena_peak_bytes = std::max(s->st.arena_peak_bytes, s->inflight_bytes);
       if (grp.empty()) continue;  // cannot happen: every request fits an empty lane (checked at submit)
@@ -908,6 +873,7 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_VARIANT: if (v < 0 || v > kLevelVariant) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
     case RCM_PARAM_PROFILE: c->profile = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_BATCH_CAP: if (v < 0) return RCM_EINVAL; c->batch_cap = v; return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
@@ -931,6 +897,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_VARIANT: *v = c->variant; return RCM_OK;
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
     case RCM_PARAM_PROFILE: *v = c->profile; return RCM_OK;
+    case RCM_PARAM_BATCH_CAP: *v = c->batch_cap; return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
@@ -1138,6 +1105,8 @@ int rcm_batch_create(int device, rcm_batch** out) {
   b->device = device;
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
   for (auto& e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&b->h_g), sizeof(ManyControl), hipHostMallocDefault) == hipSuccess;
+  if (ok) *b->h_g = ManyControl{};
   hipDeviceProp_t prop;
   ok = ok && hipGetDeviceProperties(&prop, device) == hipSuccess;
   for (int v = 0; ok && v < kVariants; ++v) {
@@ -1145,16 +1114,6 @@ int rcm_batch_create(int device, rcm_batch** out) {
     ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, many_kernel(v, false), kVarThreads[v], 0) ==
          hipSuccess;
     b->max_res[v] = std::max(1, per_cu) * prop.multiProcessorCount;
-  }
-  if (!ok) {
-    rcm_batch_destroy(b);
-    return RCM_EDEVICE;
-  }
-  if (ok) {
-    int per_cu = 0;
-    ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_order_wg<false>),
-                                                      kWgT, 0) == hipSuccess;
-    b->max_res_wg = std::max(1, per_cu) * prop.multiProcessorCount;
   }
   if (!ok) {
     rcm_batch_destroy(b);
@@ -1170,6 +1129,7 @@ int rcm_batch_destroy(rcm_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_release(b);
+  if (b->h_g) (void)hipHostFree(b->h_g);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1189,11 +1149,8 @@ int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
       batch_drop_graphs(b);
       return RCM_OK;
     case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
-    case RCM_PARAM_WG_MAX_N:
-      if (v < 0) return RCM_EINVAL;
-      b->wg_max_n = v;
-      batch_drop_graphs(b);
-      return b->count > 0 ? batch_partition(b) : RCM_OK;
+    case RCM_PARAM_WG_MAX_N:  // reserved: the one-workgroup-per-matrix engine is not part of the library
+      return v == 0 ? RCM_OK : RCM_EINVAL;
     case RCM_PARAM_VARIANT:
       if (v < 0 || v >= kVariants) return RCM_EINVAL;
       b->variant = v;
@@ -1294,9 +1251,9 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   b->jobs_host = jobs;
   RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipMemcpyAsync(b->d_nnz, nnz, C * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
-  std::vector<int32_t> blk;
-  b->nblk_order = build_blk(b, b->order.data(), count, blk);
-  RCM_HIP(hipMemcpyAsync(b->blk_order, blk.data(), (C + 1) * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  b->nblk_order = build_blk(b, b->order.data(), count, b->blk_order_h);
+  RCM_HIP(hipMemcpyAsync(b->blk_order, b->blk_order_h.data(), (C + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                         b->stream));
   RCM_HIP(hipMemsetAsync(b->scratch, 0, 3 * C * sizeof(uint32_t), b->stream));
   hipLaunchKernelGGL(k_validate_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
                      count, b->blk_order, b->d_nnz, b->scratch);
@@ -1311,7 +1268,6 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
     if (r) rc = r;
     b->maxrow[k] = v[2 * k + 1];
   }
-  if (rc == RCM_OK) rc = batch_partition(b);
   b->timing = rcm_batch_timing_t{};
   b->timing.count = count;
   b->timing.total_n = tot;
@@ -1351,11 +1307,10 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
     (void)build_blk(b, active.data(), A, blk);
     RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
                            b->stream));
-    int32_t nh = 0;  // `active` keeps the order of `order`: the helper engine's matrices come first
-    while (nh < A && !b->wg[active[nh]]) ++nh;
-    enqueue_many(b, b->d_active, b->blk_active, blk[nh], nh, A - nh, true);
+    enqueue_many(b, b->d_active, b->blk_active, blk[A], A, true);
     hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
     RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+    RCM_HIP(hipMemcpyAsync(b->h_g, b->g, sizeof(ManyControl), hipMemcpyDeviceToHost, b->stream));
     RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
     RCM_HIP(hipEventRecord(b->ev[3], b->stream));
     const int wr = batch_wait(b, b->ev[3]);
@@ -1410,7 +1365,7 @@ int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* s
   RCM_HIP(hipMemcpyAsync(b->d_starts, start, (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipEventRecord(b->ev[0], b->stream));
   auto enqueue_run = [&]() {
-    enqueue_many(b, b->d_order, b->blk_order, b->blk_order_h[b->n_helper], b->n_helper, C - b->n_helper, false);
+    enqueue_many(b, b->d_order, b->blk_order, b->nblk_order, C, false);
     if (rev)
       hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
                          C, b->blk_order);
@@ -1432,6 +1387,7 @@ int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* s
   }
   RCM_HIP(hipEventRecord(b->ev[1], b->stream));
   RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->h_g, b->g, sizeof(ManyControl), hipMemcpyDeviceToHost, b->stream));
   RCM_HIP(hipEventRecord(b->ev[3], b->stream));
   const int wr = batch_wait(b, b->ev[3]);
   if (wr) return wr;
@@ -1612,7 +1568,7 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
     rcm_batch_set_param(b, RCM_PARAM_GRAPH, 0);  // every group is a different set
     rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, kServerHelperNodes);
     wtotal += std::min(w, b->max_resident);
-    rc = batch_reserve(b, s->lane_bytes, c.slots);  // the lane's arena, allocated once
+    rc = batch_reserve(b, s->lane_bytes, c.slots, true);  // the lane's arena, allocated once, exactly
     b->arena_fixed = true;
   }
   if (rc) {
@@ -1633,8 +1589,8 @@ int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_
   if (!s || !id_out || n < 1 || n >= (1 << 30) || nnz < 0 || nnz > INT32_MAX || !row_ptr || (nnz > 0 && !col_idx) ||
       start_node >= n)
     return RCM_EINVAL;
-  const size_t bytes = job_bytes(n, nnz);
-  if (bytes + kLaneSlack > s->lane_bytes) return RCM_ENOMEM;  // can never fit a lane
+  const size_t bytes = GroupSize{}.plus(n, nnz).bytes();
+  if (bytes > s->lane_bytes) return RCM_ENOMEM;  // can never fit a lane
   auto jp = std::make_shared<ServerJob>();
   jp->n = n;
   jp->nnz = nnz;
@@ -1727,9 +1683,11 @@ static std::vector<int> tune_workers(int max_res) {
   return w;
 }
 
+// Median of the timings; +inf for none, so a candidate without timings is never chosen.
 static double median_of(std::vector<double> v) {
+  if (v.empty()) return std::numeric_limits<double>::infinity();
   std::sort(v.begin(), v.end());
-  return v.empty() ? 0.0 : v[v.size() / 2];
+  return v[v.size() / 2];
 }
 
 int rcm_ctx_autotune(rcm_ctx* c, int32_t start, int32_t repeats, rcm_tune_result* out) {
@@ -1842,13 +1800,17 @@ int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_
     int rc = rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, h);
     std::vector<double> t;
     for (int r = 0; rc == RCM_OK && r < repeats; ++r) {
-      rc = rcm_batch_run(b, start, RCM_FLAG_STABLE, st.data());
-      if (rc == RCM_OK) t.push_back(b->timing.run_ms);
+      // per-matrix failures (a disconnected member) do not void the timing: every matrix was run, as in the
+      // variant loop above; only launch-level failures end the tuning
+      const int run = rcm_batch_run(b, start, RCM_FLAG_STABLE, st.data());
+      if (run == RCM_EDEVICE || run == RCM_ETIMEOUT || run == RCM_EINVAL || run == RCM_ENOTLOADED) rc = run;
+      else t.push_back(b->timing.run_ms);
     }
-    if (rc == RCM_EDEVICE || rc == RCM_ETIMEOUT || rc == RCM_EINVAL || rc == RCM_ENOTLOADED) {
+    if (rc != RCM_OK) {
       rcm_batch_set_param(b, RCM_PARAM_HELPER_NODES, keep_h);
       return rc;
     }
+    if (t.empty()) continue;  // nothing timed: not a candidate
     ++count;
     const double ms = median_of(t);
     if (ms < best.best_ms) {

@@ -1,4 +1,8 @@
-// engine_wg.hpp -- one workgroup per matrix: the many-matrix engine for small matrices with short rows.
+// engine_wg.hpp -- one workgroup per matrix: a many-matrix engine for small matrices with short rows.
+//
+// NOT BUILT INTO THE LIBRARY. Its first GPU run (the batched start search on a small set) ended in a GPU memory
+// fault whose cause has not been shown; the library's many-matrix forms use engine_many.hpp only. This file is
+// compiled only by the CPU wave emulator tests, which keep its logic checked against the oracle.
 //
 // A persistent workgroup takes a whole matrix from the launch's order (atomicAdd ticket, largest first) and
 // builds its order level by level with no other workgroup involved, so no batch records, look-back or
@@ -15,7 +19,7 @@
 //      the chunk's earlier parents -- the stable valence sort of the serial algorithm.
 //
 // The rows of a matrix given to this engine are at most kWgMaxRow long (checked on the host), so any single
-// parent fits a chunk. Device code.
+// parent fits a chunk.
 #pragma once
 #include "engine_many.hpp"
 

@@ -1,5 +1,5 @@
 // wave_emu.hpp -- CPU emulation of the gfx950 wave64 execution model, for running the engine's device code
-// (parallelbatchrcm_amd/csrc/engine.hpp and engine_wave.hpp) on the host in tests. TEST INFRASTRUCTURE ONLY.
+// (parallelbatchrcm_amd/csrc/engine.hpp and tests/emu/engine_wave.hpp) on the host in tests. TEST INFRASTRUCTURE ONLY.
 //
 // Every lane is a host thread. Cross-lane operations (ballot, shuffles, readlane, DPP moves) are collectives:
 // each lane deposits its value, the 64 lanes of the wave meet at a barrier, each reads what it needs, and they

@@ -54,6 +54,13 @@ def orc() -> C.CDLL:
         L.orc_verify_order.argtypes = [C.c_int32, I32P, I32P, C.c_int32, I32P, C.c_int32, C.c_int32]
         L.orc_prepare.restype = C.c_int64
         L.orc_prepare.argtypes = [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.POINTER(I32P), C.POINTER(I32P), I64P]
+        DP = C.POINTER(C.c_double)
+        L.orc_prepare_values.restype = C.c_int64
+        L.orc_prepare_values.argtypes = [C.c_int32, C.c_int64, I32P, I32P, DP, C.c_int32, C.POINTER(I32P),
+                                         C.POINTER(I32P), C.POINTER(DP), I64P]
+        L.orc_convert.restype = C.c_int64
+        L.orc_convert.argtypes = [C.c_int32, C.c_int64, I32P, I32P, DP, C.POINTER(I32P), C.POINTER(I32P),
+                                  C.POINTER(DP)]
         L.orc_free.restype = None
         L.orc_free.argtypes = [C.c_void_p]
         _orc = L
@@ -167,6 +174,46 @@ def prepare(rp, ci, check_sym):
     orc().orc_free(C.cast(orp, C.c_void_p))
     orc().orc_free(C.cast(oci, C.c_void_p))
     return int(rc), r, c
+
+
+def prepare_values(rp, ci, vals, check_sym):
+    """Oracle prepareRCM with float64 values: (status_or_added, rp, ci, vals)."""
+    rp, ci = as_csr(rp, ci)
+    v = np.ascontiguousarray(vals, np.float64)
+    n = rp.shape[0] - 1
+    orp, oci, ov, onnz = I32P(), I32P(), C.POINTER(C.c_double)(), C.c_int64()
+    rc = orc().orc_prepare_values(n, ci.shape[0], _p32(rp), _p32(ci), v.ctypes.data_as(C.POINTER(C.c_double)),
+                                  1 if check_sym else 0, C.byref(orp), C.byref(oci), C.byref(ov), C.byref(onnz))
+    if rc < 0:
+        return int(rc), None, None, None
+    m = onnz.value
+    r = np.ctypeslib.as_array(orp, shape=(n + 1,)).copy()
+    c = np.ctypeslib.as_array(oci, shape=(max(m, 1),))[:m].copy()
+    w = np.ctypeslib.as_array(ov, shape=(max(m, 1),))[:m].copy()
+    for p in (orp, oci, ov):
+        orc().orc_free(C.cast(p, C.c_void_p))
+    return int(rc), r, c, w
+
+
+def convert(rows, row_ids, col_ids, vals=None):
+    """Oracle convert (COO -> CSR sorted by (row, col), duplicates in input order): (rp, ci, vals or None)."""
+    r = np.ascontiguousarray(row_ids, np.int32)
+    c = np.ascontiguousarray(col_ids, np.int32)
+    v = None if vals is None else np.ascontiguousarray(vals, np.float64)
+    orp, oci, ov = I32P(), I32P(), C.POINTER(C.c_double)()
+    m = orc().orc_convert(int(rows), r.shape[0], _p32(r), _p32(c),
+                          None if v is None else v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(orp), C.byref(oci),
+                          C.byref(ov))
+    assert m >= 0
+    rp = np.ctypeslib.as_array(orp, shape=(rows + 1,)).copy()
+    ci = np.ctypeslib.as_array(oci, shape=(max(m, 1),))[:m].copy()
+    orc().orc_free(C.cast(orp, C.c_void_p))
+    orc().orc_free(C.cast(oci, C.c_void_p))
+    w = None
+    if v is not None:
+        w = np.ctypeslib.as_array(ov, shape=(max(m, 1),))[:m].copy()
+        orc().orc_free(C.cast(ov, C.c_void_p))
+    return rp, ci, w
 
 
 def ref_rcm(rp, ci, start, threads=1, batchsize=1, stable=True):

@@ -197,8 +197,9 @@ def test_batch_autotune_keeps_results():
         assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, int(starts[k]))), k
 
 
-@pytest.mark.skip(reason="the one-workgroup-per-matrix engine (engine_wg.hpp) faulted the GPU on its first run "
-                         "with no cause shown; it is off by default and tested only in the CPU emulator")
+@pytest.mark.skip(reason="the one-workgroup-per-matrix engine (tests/emu/engine_wg.hpp) faulted the GPU on its "
+                         "first run with no cause shown; it is not built into the library (wg_max_n > 0 is "
+                         "rejected) and is tested only in the CPU emulator")
 @pytest.mark.parametrize("wg_max_n", [0, 1 << 18, 1 << 30])
 def test_batch_one_workgroup_engine_matches_oracle(wg_max_n):
     """Matrices with short rows go to one workgroup each (RCM_PARAM_WG_MAX_N), the rest (a hub graph, an R-MAT)
@@ -256,6 +257,9 @@ def test_batch_helper_cap_keeps_results():
             got[h] = b.perms()
         with pytest.raises(Exception):
             b.set_param("helper_nodes", 0)
+        b.set_param("wg_max_n", 0)  # reserved knob: only 0 (off) is accepted
+        with pytest.raises(P.RcmError):
+            b.set_param("wg_max_n", 1)
     for k in range(len(bs)):
         want = O.cm(bs.matrix(k).row_offsets, bs.matrix(k).col_ids, int(starts[k]))
         for h, perms in got.items():

@@ -1,7 +1,8 @@
 """GPU parity of the steps either side of the reorder (include/rcm_matrix_ops.h), through the C ABI:
 
-* canonicalisation (COO -> CSR + prepareRCM) against the host convert / prepareRCM, which tests/test_formats.py
-  pins to the reference's own code -- bit-exact, values included;
+* canonicalisation (COO -> CSR + prepareRCM) against the oracle's convert and value-carrying prepareRCM
+  (oracle/rcm_oracle.c, pinned to the reference's own Graph<double>::prepareRCM in tests/test_oracle.py) --
+  bit-exact, values included;
 * B = P A P^T against its numpy definition (tests/matrix_ref.py) -- bit-exact, values included;
 * the whole device pipeline (COO in HBM -> canonicalise -> start search -> reorder -> permute) against the host
   pipeline built from the oracle.
@@ -32,6 +33,14 @@ def ops():
     o = P.MatrixOps()
     yield o
     o.close()
+
+
+def oracle_canonical(n, r, c, v, check_sym):
+    """The oracle's convert + prepareRCM with values: (CSR, links)."""
+    rp, ci, vals = O.convert(n, r, c, v)
+    added, orp, oci, ov = O.prepare_values(rp, ci, vals, check_sym)
+    assert added >= 0
+    return P.CSR(orp, oci, ov), added
 
 
 def long_row_matrix(rng, n, k, hubs=2, hub_len=3000):
@@ -97,7 +106,7 @@ def test_canonicalize_matches_host(ops, n, comps, check_sym):
         r, c, v = M.rows_of(sym.row_offsets).astype(np.int32), sym.col_ids, sym.data
     _say("canonicalize", n, r.size, comps, check_sym)
     got, links = ops.canonicalize(n, n, r, c, v, check_sym)
-    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), check_sym)
+    host, hlinks = oracle_canonical(n, r, c, v, check_sym)
     assert links == hlinks
     np.testing.assert_array_equal(got.row_offsets, host.row_offsets)
     np.testing.assert_array_equal(got.col_ids, host.col_ids)
@@ -160,7 +169,7 @@ class _Hip:
 
 def test_device_pipeline_matches_host(ops):
     """COO in HBM -> canonicalise -> start search -> reorder (RCM) -> B = P A P^T, with no host round trip between
-    the steps; compared with the host pipeline convert + prepareRCM + oracle CM + the permutation definition."""
+    the steps; compared with the oracle's convert + prepareRCM + CM and the permutation definition."""
     rng = np.random.default_rng(21)
     side = 40
     grid = P.workloads.stencil2d(side, side)
@@ -210,8 +219,8 @@ def test_device_pipeline_matches_host(ops):
         if ctx:
             lib.rcm_ctx_destroy(ctx)
         hip.close()
-    # host pipeline
-    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), True)
+    # the oracle's pipeline
+    host, hlinks = oracle_canonical(n, r, c, v, True)
     assert links.value == hlinks == 3
     rc, hstart, _, _ = O.peripheral(host.row_offsets, host.col_ids)
     assert rc == 0 and hstart == start.value
@@ -228,7 +237,7 @@ def test_device_pipeline_matches_host(ops):
 @pytest.mark.slow
 def test_full_size_permute_and_canonicalize(ops):
     """128^3 7-point matrix (2,097,152 rows, 14,581,760 entries): canonicalise its shuffled lower triangle, reorder,
-    permute; checked against the host convert + prepareRCM and the permutation definition."""
+    permute; checked against the oracle's convert + prepareRCM and the permutation definition."""
     rng = np.random.default_rng(5)
     m = P.workloads.config(2, seed=7)
     n = m.rows
@@ -241,7 +250,7 @@ def test_full_size_permute_and_canonicalize(ops):
     v = rng.standard_normal(r.size).astype(np.float32).astype(np.float64)
     _say("full size canonicalize", n, r.size)
     got, links = ops.canonicalize(n, n, r, c, v, True)
-    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), True)
+    host, hlinks = oracle_canonical(n, r, c, v, True)
     assert links == hlinks
     np.testing.assert_array_equal(got.row_offsets, host.row_offsets)
     np.testing.assert_array_equal(got.col_ids, host.col_ids)

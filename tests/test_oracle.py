@@ -251,3 +251,74 @@ def test_prepare_connects_components_in_order():
     rows = {i: set(c2[r2[i]:r2[i + 1]].tolist()) for i in range(8)}
     assert 1 in rows[2] and 3 in rows[4] and 4 in rows[5]
     assert O.cm(r2, c2, 0).shape[0] == 8
+
+
+def _coo_without_duplicates(rng, n, k, sym):
+    r = rng.integers(0, n, k)
+    c = rng.integers(0, n, k)
+    if sym:
+        r, c = np.concatenate([r, c]), np.concatenate([c, r])
+    r = np.concatenate([r, np.arange(n)])  # full diagonal (the repair loop needs it)
+    c = np.concatenate([c, np.arange(n)])
+    _, first = np.unique(r.astype(np.int64) * n + c, return_index=True)
+    keep = np.zeros(r.size, bool)
+    keep[first] = True  # no duplicates: the reference's unstable sort then has a unique result
+    order = rng.permutation(int(keep.sum()))
+    return r[keep][order].astype(np.int32), c[keep][order].astype(np.int32)
+
+
+@needs_ref
+@pytest.mark.parametrize("seed,sym", [(11, True), (12, False), (13, False), (14, True)])
+def test_oracle_prepare_values_equals_reference(seed, sym):
+    """orc_prepare_values against the reference's own Graph<double>::prepareRCM with values."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(50, 400))
+    r, c = _coo_without_duplicates(rng, n, n // 3, sym)
+    v = rng.standard_normal(r.size)
+    rp, ci, vals = O.convert(n, r, c, v)
+    for check_sym in (False, True):
+        if not check_sym and not sym:
+            continue  # the reference's repair on a directed pattern is covered by test_oracle_prepare_equals_reference
+        rc, orp, oci, ov = O.prepare_values(rp, ci, vals, check_sym)
+        assert rc >= 0
+        _, _, rrp, rci, rv = O.ref_prepare_values(n, n, rp, ci, vals, check_sym, True)
+        np.testing.assert_array_equal(orp, rrp)
+        np.testing.assert_array_equal(oci, rci)
+        np.testing.assert_array_equal(ov, rv)
+        rc2, prp, pci = O.prepare(rp, ci, check_sym)  # the pattern-only form agrees
+        assert rc2 == rc
+        np.testing.assert_array_equal(prp, orp)
+        np.testing.assert_array_equal(pci, oci)
+
+
+def test_oracle_prepare_values_closed_form():
+    """Two one-sided entries get their transposes with the same value; three components get two links of value 1."""
+    # (0,1) stored once, (3,2) stored once; diagonal everywhere; components {0,1}, {2,3}, {4}
+    r = np.array([0, 0, 1, 2, 3, 3, 4], np.int32)
+    c = np.array([0, 1, 1, 2, 2, 3, 4], np.int32)
+    v = np.array([5.0, -7.0, 6.0, 8.0, -9.0, 4.0, 3.0])
+    rp, ci, vals = O.convert(5, r, c, v)
+    rc, orp, oci, ov = O.prepare_values(rp, ci, vals, True)
+    assert rc == 2
+    want = {(0, 0): 5.0, (0, 1): -7.0, (1, 0): -7.0, (1, 1): 6.0, (1, 2): 1.0, (2, 1): 1.0, (2, 2): 8.0,
+            (2, 3): -9.0, (3, 2): -9.0, (3, 3): 4.0, (3, 4): 1.0, (4, 3): 1.0, (4, 4): 3.0}
+    got = {(i, int(oci[k])): float(ov[k]) for i in range(5) for k in range(orp[i], orp[i + 1])}
+    assert got == want
+
+
+def test_oracle_convert_is_stable_lexicographic():
+    """orc_convert equals a stable numpy lexsort on (row, col), duplicates in input order with their values."""
+    rng = np.random.default_rng(15)
+    n, k = 300, 6000
+    r = rng.integers(0, n, k).astype(np.int32)
+    c = rng.integers(0, 40, k).astype(np.int32)
+    v = np.arange(k, dtype=np.float64)  # distinct values expose any reordering of duplicates
+    rp, ci, vals = O.convert(n, r, c, v)
+    order = np.lexsort((np.arange(k), c, r))
+    np.testing.assert_array_equal(ci, c[order])
+    np.testing.assert_array_equal(vals, v[order])
+    np.testing.assert_array_equal(rp, np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))]))
+    rp2, ci2, none = O.convert(n, r, c)
+    assert none is None
+    np.testing.assert_array_equal(rp2, rp)
+    np.testing.assert_array_equal(ci2, ci)
