@@ -310,6 +310,28 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   return RCM_OK;
 }
 
+// One step of SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:44-90, outputnodes = 32) on the last
+// w = min(32, n - 1) nodes a plain-BFS sweep discovered, in discovery order (node, BFS level, row length).
+// Returns true when the sweep's eccentricity equals the previous one (the start *cur is final); otherwise moves
+// *cur to the node of least row length among those from the first of greatest level on, later nodes winning
+// ties, and records the eccentricity in *lastdist.
+bool peripheral_step(const int32_t* nodes, const int32_t* dists, const int32_t* degs, int32_t w, uint32_t* lastdist,
+                     int32_t* cur) {
+  if ((uint32_t)dists[w - 1] == *lastdist) return true;
+  int32_t idx = 0;  // first index holding the maximum distance
+  for (int32_t i = 1; i < w; ++i)
+    if (dists[i] > dists[idx]) idx = i;
+  int64_t minval = INT64_MAX;
+  for (int32_t i = idx; i < w; ++i) {  // minimum degree, later nodes win ties
+    if ((int64_t)degs[i] <= minval) {
+      minval = degs[i];
+      *cur = nodes[i];
+    }
+  }
+  *lastdist = (uint32_t)dists[w - 1];
+  return false;
+}
+
 int load_common(rcm_ctx* c, int32_t n, int64_t nnz) {
   int rc = ensure_state(c, n);
   if (rc) return rc;
@@ -982,30 +1004,19 @@ int rcm_ctx_find_start(rcm_ctx* c, int32_t* start_out, int32_t* iterations_out) 
   } else {
     uint32_t lastdist = 0;
     const int32_t w = std::min<int32_t>(32, n - 1);
-    std::vector<int32_t> nodes((size_t)w), dists((size_t)w);
-    std::vector<uint64_t> rows((size_t)w);
+    int32_t nodes[32], dists[32], degs[32];
+    uint64_t rows[32];
     for (;;) {
       ++iters;
       c->have_perm = false;
       const int rc = run_engine(c, cur, true);
       if (rc) return rc;
-      RCM_HIP(hipMemcpyAsync(nodes.data(), c->perm + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-      RCM_HIP(hipMemcpyAsync(dists.data(), c->level + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-      RCM_HIP(hipMemcpyAsync(rows.data(), c->qinfo + (n - w), (size_t)w * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipMemcpyAsync(nodes, c->perm + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipMemcpyAsync(dists, c->level + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipMemcpyAsync(rows, c->qinfo + (n - w), (size_t)w * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
       RCM_HIP(hipStreamSynchronize(c->stream));
-      if ((uint32_t)dists[w - 1] == lastdist) break;
-      int32_t idx = 0;  // first index holding the maximum distance
-      for (int32_t i = 1; i < w; ++i)
-        if (dists[i] > dists[idx]) idx = i;
-      int64_t minval = INT64_MAX;
-      for (int32_t i = idx; i < w; ++i) {  // minimum degree, later nodes win ties
-        const int64_t val = (int64_t)(uint32_t)(rows[i] >> 32) - (int64_t)(uint32_t)rows[i];
-        if (val <= minval) {
-          minval = val;
-          cur = nodes[i];
-        }
-      }
-      lastdist = (uint32_t)dists[w - 1];
+      for (int32_t i = 0; i < w; ++i) degs[i] = (int32_t)((uint32_t)(rows[i] >> 32) - (uint32_t)rows[i]);
+      if (peripheral_step(nodes, dists, degs, w, &lastdist, &cur)) break;
     }
   }
   c->timing.start_search_ms =
@@ -1323,20 +1334,7 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
       if (status[k] != RCM_OK) continue;
       const int32_t w = std::min<int32_t>(32, b->n[k] - 1);
       const int32_t* nodes = b->h_tail + (size_t)i * 96;
-      const int32_t* dists = nodes + 32;
-      const int32_t* degs = nodes + 64;
-      if ((uint32_t)dists[w - 1] == lastdist[k]) continue;  // converged: cur[k] is the start
-      int32_t idx = 0;  // first index holding the maximum distance
-      for (int32_t t = 1; t < w; ++t)
-        if (dists[t] > dists[idx]) idx = t;
-      int64_t minval = INT64_MAX;
-      for (int32_t t = idx; t < w; ++t) {  // minimum degree, later nodes win ties
-        if ((int64_t)degs[t] <= minval) {
-          minval = degs[t];
-          cur[k] = nodes[t];
-        }
-      }
-      lastdist[k] = (uint32_t)dists[w - 1];
+      if (peripheral_step(nodes, nodes + 32, nodes + 64, w, &lastdist[k], &cur[k])) continue;  // converged
       next.push_back(k);
     }
     active.swap(next);

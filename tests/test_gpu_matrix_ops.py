@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 import matrix_ref as M
+from hip_buffers import HipBuffers
 import oracle_lib as O
 import parallelbatchrcm_amd as P
 from parallelbatchrcm_amd import _native
@@ -130,43 +131,6 @@ def test_canonicalize_errors(ops):
     assert links == 2
 
 
-class _Hip:
-    """Device buffers through the HIP runtime the engine library itself loaded (one runtime per process)."""
-
-    def __init__(self):
-        _native.lib()
-        L = C.CDLL("libamdhip64.so.7")
-        L.hipMalloc.restype = C.c_int
-        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        L.hipFree.restype = C.c_int
-        L.hipFree.argtypes = [C.c_void_p]
-        L.hipMemcpy.restype = C.c_int
-        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.L = L
-        self.bufs = []
-
-    def alloc(self, nbytes):
-        p = C.c_void_p()
-        assert self.L.hipMalloc(C.byref(p), max(int(nbytes), 1)) == 0
-        self.bufs.append(p)
-        return p
-
-    def upload(self, a):
-        p = self.alloc(a.nbytes)
-        assert self.L.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) == 0  # host to device
-        return p
-
-    def download(self, p, count, dtype):
-        a = np.empty(count, dtype)
-        assert self.L.hipMemcpy(a.ctypes.data_as(C.c_void_p), p, a.nbytes, 2) == 0  # device to host
-        return a
-
-    def close(self):
-        for p in self.bufs:
-            self.L.hipFree(p)
-        self.bufs = []
-
-
 def test_device_pipeline_matches_host(ops):
     """COO in HBM -> canonicalise -> start search -> reorder (RCM) -> B = P A P^T, with no host round trip between
     the steps; compared with the oracle's convert + prepareRCM + CM and the permutation definition."""
@@ -187,7 +151,7 @@ def test_device_pipeline_matches_host(ops):
     r, c, v = r[shuffle], c[shuffle], v[shuffle]
     _say("device pipeline", n, r.size)
     lib = _native.lib()
-    hip = _Hip()
+    hip = HipBuffers()
     ctx = C.c_void_p()
     try:
         d_r, d_c, d_v = hip.upload(r), hip.upload(c), hip.upload(v)
