@@ -47,6 +47,10 @@ __device__ __forceinline__ uint32_t st_b(uint64_t w) { return (uint32_t)(w & 0x3
 
 enum : uint32_t { ERR_NONE = 0, ERR_TIMEOUT = 1, ERR_CAPACITY = 2, ERR_LONGROW = 3 };
 
+// Batch range records are begin | end << 32 (both < 2^30). A record with this bit set is kept by the workgroup
+// that produced it: the holder of its ticket does not process it and takes the next ticket.
+constexpr uint64_t kKeptBit = 1ull << 63;
+
 // The ticket (an atomic per batch), the progress counter (an atomic per batch) and the done word (polled by idle
 // workers) sit on separate 128-byte lines, so the atomics and polls on one do not queue behind the others.
 struct alignas(128) Control {
@@ -143,6 +147,9 @@ struct Smem {
   uint32_t cinfo[2 * (G::S / kWave)];  // per 64-slot chunk: count, degree sum (compaction and cut exchanges)
   uint32_t wtmp[2 * G::W + 2];
   int32_t b, s, e, early, quit;
+  int32_t kept;  // the batch in b / s / e is the previous batch's first child, its parents staged in key[kfin]
+  int32_t kfin;
+  uint32_t tick;  // the ticket counter, read during the look-back
   uint32_t cnt, nb, excl_off, excl_nb, maxdeg, maxpdeg;
   static_assert(HS >= 2 * G::P, "segment starts and degree bases of every parent fit in hist");
   static_assert(G::S == G::W * kWave * G::EPT, "compaction rounds cover the slots");
@@ -681,19 +688,35 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     return;
   }
 
-  // A. parents (written by the producing batch with sc1 stores) and the non-blocking claim check
+  // A. parents (written by the producing batch with sc1 stores, or staged in LDS when this workgroup produced
+  //    them and kept the batch) and the non-blocking claim check. A kept batch skips the check: its
+  //    predecessors were still claiming a moment ago, and the non-early path is correct in every case.
+  const bool kept = sm.kept != 0;
   uint32_t d = 0;
   if (tid < m) {
-    const int32_t p = ld_sync(&a.perm[s + tid]);
-    const uint64_t q = ld_sync(&a.qinfo[s + tid]);
-    const uint32_t rb = (uint32_t)q, re = (uint32_t)(q >> 32);
+    int32_t p;
+    uint32_t rb, re;
+    int32_t lev = 0;
+    if (kept) {
+      const uint64_t k0 = sm.key[sm.kfin][tid], k1 = sm.key[sm.kfin][G::P + tid];
+      p = (int32_t)(uint32_t)k0;
+      rb = (uint32_t)(k0 >> 32);
+      re = (uint32_t)k1;
+      lev = (int32_t)(uint32_t)(k1 >> 32);
+    } else {
+      p = ld_sync(&a.perm[s + tid]);
+      const uint64_t q = ld_sync(&a.qinfo[s + tid]);
+      rb = (uint32_t)q;
+      re = (uint32_t)(q >> 32);
+      if constexpr (BFS) lev = ld_sync(&a.level[s + tid]);
+    }
     sm.pnode[tid] = p;
     sm.prb[tid] = rb;
     d = re - rb;
-    if constexpr (BFS) sm.plevel[tid] = ld_sync(&a.level[s + tid]);
+    if constexpr (BFS) sm.plevel[tid] = lev;
   }
   if (wave_id() == 0) {
-    const bool ok = claims_complete_before(a, (int64_t)b);
+    const bool ok = !kept && claims_complete_before(a, (int64_t)b);
     if (lane_id() == 0) sm.early = ok ? 1 : 0;
   }
   // Slot -> parent marks for the claims: si[first slot of parent i] = i + 1, 0 on every other slot, and for each
@@ -711,7 +734,10 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   uint32_t D, maxpdeg;
   const uint32_t soff = block_excl_scan_max<T>(d, d, sm.wtmp, &D, &maxpdeg);
   if (tid < m) sm.so[tid] = soff;
-  if (tid == 0) sm.so[m] = D;
+  if (tid == 0) {
+    sm.so[m] = D;
+    sm.kept = 0;  // every thread read it before the scan's barriers; set again when a child batch is kept
+  }
   if (tid < m && d > 0) {
     if (soff < (uint32_t)S) sm.si[soff] = (uint16_t)(tid + 1);
     for (uint32_t c = (soff + kWave - 1) / kWave; c < (uint32_t)(S / kWave) && c * kWave < soff + d; ++c)
@@ -1023,15 +1049,19 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   }
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CUT] += _t - pf[15]; pf[15] = _t; } }
-  // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record
+  // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record. The
+  //    ticket counter is read beside the look-back (one round trip for both): it tells whether the first child
+  //    batch's ticket has been handed out already.
   if (tid == 0) st_sync(&a.bstat[b], mk_status(ST_AGG, cnt, nb));
   if (wave_id() == 0) {
+    const uint32_t tick = wave_uniform(ld_sync(&a.ctl->ticket));
     uint32_t xo = 0, xb = 0;
     const bool ok = lookback_prefix(a, (int64_t)b, deadline, &xo, &xb);
     if (lane_id() == 0) {
       sm.quit = ok ? 0 : 1;
       sm.excl_off = xo;
       sm.excl_nb = xb;
+      sm.tick = tick;
       if (ok) st_sync(&a.bstat[b], mk_status(ST_INC, xo + cnt, xb + nb));
     }
   }
@@ -1045,20 +1075,35 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   }
 
   if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PREFIX] += _t - pf[15]; pf[15] = _t; } }
-  // G. write children (queue entry, row bounds, level), then publish the child batch records
+  // G. write children (queue entry, row bounds, level), then publish the child batch records.
+  //    Work-first: when the first child batch's ticket has already been handed out (so every batch below it
+  //    is held by a running workgroup, and waiting on them cannot deadlock), this workgroup keeps that batch
+  //    and processes it next with its parents staged in LDS (key[fin], free after the cut): no wait for the
+  //    record and no reload of the parents. Its record is published with kKeptBit set, so the ticket's holder
+  //    takes another ticket instead.
+  const bool keep = nb > 0 && sm.tick > xb;
+  const uint32_t kend = nb > 1 ? (uint32_t)cut[1] : cnt;  // the first child batch: children [0, kend)
   for (uint32_t o = tid; o < cnt; o += T) {
     const uint32_t j = sm.val[fin][o];
     const uint32_t p = xo + o;
-    st_sync(&a.perm[p], sm.sc[j]);
-    st_sync(&a.qinfo[p], (uint64_t)sm.srb[j] | ((uint64_t)sm.sre[j] << 32));
-    if constexpr (BFS) st_sync(&a.level[p], sm.plevel[sm.si[j]] + 1);
+    const int32_t child = sm.sc[j];
+    const uint32_t crb = sm.srb[j], cre = sm.sre[j];
+    const int32_t clev = BFS ? sm.plevel[sm.si[j]] + 1 : 0;
+    st_sync(&a.perm[p], child);
+    st_sync(&a.qinfo[p], (uint64_t)crb | ((uint64_t)cre << 32));
+    if constexpr (BFS) st_sync(&a.level[p], clev);
+    if (keep && o < kend) {
+      sm.key[fin][o] = (uint64_t)(uint32_t)child | ((uint64_t)crb << 32);
+      sm.key[fin][G::P + o] = (uint64_t)cre | ((uint64_t)(uint32_t)clev << 32);
+    }
   }
   publish_wait<STRICT>();
   __syncthreads();
   for (uint32_t t = tid; t < nb; t += T) {
     const uint32_t beg = (uint32_t)cut[t];
     const uint32_t end = t + 1 < nb ? (uint32_t)cut[t + 1] : cnt;
-    st_sync(&a.brange[xb + t], (uint64_t)(xo + beg) | ((uint64_t)(xo + end) << 32));
+    const uint64_t kb = (t == 0 && keep) ? kKeptBit : 0ull;
+    st_sync(&a.brange[xb + t], kb | (uint64_t)(xo + beg) | ((uint64_t)(xo + end) << 32));
   }
   if (tid == 0) {
     const bool final_batch = xb + nb == b + 1;  // no batch beyond b exists or can be created
@@ -1070,6 +1115,13 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       }
       const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
       if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) mark_done(a);
+    }
+    if (keep) {  // the caller processes this batch next, without acquiring it (read after its barrier)
+      sm.kept = 1;
+      sm.kfin = fin;
+      sm.b = (int32_t)xb;
+      sm.s = (int32_t)xo;
+      sm.e = (int32_t)(xo + kend);
     }
     if constexpr (PROF) {
       const uint64_t _t = wall_ticks();
@@ -1094,37 +1146,47 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   if constexpr (PROF) {
     for (int i = 0; i < kProfSlots; ++i) pf[i] = 0;
   }
+  if (tid == 0) sm.kept = 0;
+  __syncthreads();
   for (;;) {
-    if (tid == 0) {
-      if constexpr (PROF) pf[15] = wall_ticks();
-      const uint32_t b = atomic_add_u32(&a.ctl->ticket, 1u);
-      int quit = 0;
-      uint64_t r = 0;
-      for (uint32_t spin = 0;; ++spin) {
-        if (b < (uint32_t)a.bcap) {
-          r = ld_sync(&a.brange[b]);
-          if (r) break;
-        }
-        if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done word rarely (measured faster)
-          if (ld_sync(&a.ctl->done)) { quit = 1; break; }
-          if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      sm.quit = quit;
-      sm.b = (int32_t)b;
-      sm.s = (int32_t)(uint32_t)r;
-      sm.e = (int32_t)(r >> 32);
-    }
-    if constexpr (PROF) {
+    if (!sm.kept) {  // acquire: the next ticket whose batch is not kept by its producer
       if (tid == 0) {
-        const uint64_t _t = wall_ticks();
-        if (!sm.quit) pf[PF_ACQUIRE] += _t - pf[15];
-        pf[15] = _t;
+        if constexpr (PROF) pf[15] = wall_ticks();
+        int quit = 0;
+        uint32_t b = 0;
+        uint64_t r = 0;
+        for (;;) {
+          b = atomic_add_u32(&a.ctl->ticket, 1u);
+          for (uint32_t spin = 0;; ++spin) {
+            if (b < (uint32_t)a.bcap) {
+              r = ld_sync(&a.brange[b]);
+              if (r) break;
+            }
+            if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done word rarely (measured faster)
+              if (ld_sync(&a.ctl->done)) { quit = 1; break; }
+              if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          if (quit || !(r & kKeptBit)) break;
+        }
+        sm.quit = quit;
+        sm.b = (int32_t)b;
+        sm.s = (int32_t)(uint32_t)r;
+        sm.e = (int32_t)(r >> 32);
       }
+      if constexpr (PROF) {
+        if (tid == 0) {
+          const uint64_t _t = wall_ticks();
+          if (!sm.quit) pf[PF_ACQUIRE] += _t - pf[15];
+          pf[15] = _t;
+        }
+      }
+      __syncthreads();
+      if (sm.quit) break;
+    } else if constexpr (PROF) {
+      if (tid == 0) pf[15] = wall_ticks();
     }
-    __syncthreads();
-    if (sm.quit) break;
     consume_fence<STRICT>();
     process_batch<G, DIGIT, STRICT, BFS, PROF>(a, sm, deadline, pf);
     __syncthreads();

@@ -113,7 +113,8 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
       return;
     }
     if (cur >= 0) {
-      // helping job cur: the next ticket, then wait for its batch (or for the job to finish)
+      // helping job cur: the next ticket, then wait for its batch (or for the job to finish); a batch kept by
+      // its producer is skipped for the next ticket
       const JobDesc& J = m.jobs[cur];
       uint32_t t = 0;
       if (lane == 0) t = atomic_add_u32(&J.ctl->ticket, 1u);
@@ -122,6 +123,12 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
         uint64_t r = 0;
         if (t <= (uint32_t)J.n) r = ld_sync(&J.brange[t]);
         r = ((uint64_t)wave_uniform((uint32_t)(r >> 32)) << 32) | wave_uniform((uint32_t)r);
+        if (r & kKeptBit) {
+          if (lane == 0) t = atomic_add_u32(&J.ctl->ticket, 1u);
+          t = wave_uniform(t);
+          spin = 0;
+          continue;
+        }
         if (r) {
           if (lane == 0) {
             ms.cur = cur;
@@ -205,10 +212,14 @@ __global__ __launch_bounds__(G::T) void k_reorder_many(ManyArgs m) {
     ms.cursor = (uint32_t)blockIdx.x * 13u;
   }
   __syncthreads();
+  if (tid == 0) ms.s.kept = 0;
+  __syncthreads();
   for (;;) {
-    if (wave_id() == 0) find_work<G, DIGIT>(m, ms, deadline);
-    __syncthreads();
-    if (ms.s.quit) break;
+    if (!ms.s.kept) {  // a kept batch (the previous batch's first child, same job) needs no search
+      if (wave_id() == 0) find_work<G, DIGIT>(m, ms, deadline);
+      __syncthreads();
+      if (ms.s.quit) break;
+    }
     consume_fence<STRICT>();
     const RunArgs a = job_args(m, ms.cur, BFS);
     process_batch<G, DIGIT, STRICT, BFS, false>(a, ms.s, deadline, nullptr);
