@@ -61,6 +61,19 @@ enum rcm_status {
 int rcm_gpu_batch(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int32_t start_node,
                   uint32_t flags, int32_t* perm_out, int64_t* bandwidth_out, int32_t* start_node_out);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): rcm_gpu_batch on a CSR already in device
+ * memory (d_row_ptr[n+1], d_col_idx[nnz]) writing the order to the caller's device buffer d_perm_out[n]. It
+ * calls no hipMalloc, hipFree or other allocating routine: every other buffer the engine needs lives in one
+ * fixed file-scope device array (72 MiB + 1 KiB per device that runs it, held until the process ends), so n is
+ * at most 2,097,152; larger n returns RCM_ECAPACITY. Runs on the current device's default stream and returns
+ * when the result is in d_perm_out. start_node, flags, bandwidth_out and start_node_out (host, nullable) as in
+ * rcm_gpu_batch. Every call on one device shares that device's array: calls of one process are serialised,
+ * calls from different processes on one device race. */
+int rcm_gpu_batch_noalloc(int32_t n, int64_t nnz, const int32_t* d_row_ptr, const int32_t* d_col_idx,
+                          int32_t start_node, uint32_t flags, int32_t* d_perm_out, int64_t* bandwidth_out,
+                          int32_t* start_node_out);
+
 /* ---- device-resident context ---- */
 typedef struct rcm_ctx rcm_ctx;
 

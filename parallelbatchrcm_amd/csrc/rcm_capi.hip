@@ -781,6 +781,89 @@ void lane_loop(rcm_server* s, int lane) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------------------
+// The one-shot reorder that allocates nothing (rcm_gpu_batch_noalloc): rcm_gpu_batch on a CSR already in device
+// memory, with every buffer the engine needs beyond the caller's input and output in one file-scope device
+// array, `scratch`. Its size is fixed: matrices of at most kNoAllocMaxN rows. Each device holds its own copy
+// until the process ends; its address is looked up once per device with hipGetSymbolAddress and only that
+// address is used on the host.
+// ---------------------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int32_t kNoAllocMaxN = 1 << 21;  // 2,097,152 rows (the 128^3 stencil)
+constexpr int kNoAllocDevices = 64;
+
+// Byte offsets of the engine buffers of an n-row matrix in the array, each 256-byte aligned:
+// claim u64[n] (the inverse permutation for the bandwidth after the run), qinfo u64[n], bstat u64[n+1],
+// brange u64[n+1], level i32[n], the control block, 16 words (validation flags and longest row, bandwidth).
+struct NoAllocLayout {
+  size_t claim, qinfo, bstat, brange, level, ctl, words, total;
+};
+constexpr size_t al256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr NoAllocLayout noalloc_layout(size_t n) {
+  NoAllocLayout L{};
+  L.claim = 0;
+  L.qinfo = L.claim + al256(8 * n);
+  L.bstat = L.qinfo + al256(8 * n);
+  L.brange = L.bstat + al256(8 * (n + 1));
+  L.level = L.brange + al256(8 * (n + 1));
+  L.ctl = L.level + al256(4 * n);
+  L.words = L.ctl + al256(sizeof(Control));
+  L.total = L.words + al256(16 * sizeof(uint32_t));
+  return L;
+}
+constexpr size_t kNoAllocScratchBytes = noalloc_layout((size_t)kNoAllocMaxN).total;  // 72 MiB + 1 KiB
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+struct NoAllocDevice {
+  std::mutex mu;                  // calls of this process on one device run one at a time
+  unsigned char* base = nullptr;  // the device address of `scratch` on this device
+  int max_resident = 0;           // co-resident workgroups of the variant-0 reorder kernel
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+__global__ void k_fill_u32(uint32_t* p, int32_t count, uint32_t v) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < count) p[i] = v;
+}
+
+// perm reversed in place: thread i swaps entries i and n - 1 - i.
+__global__ void k_reverse_inplace(int32_t* perm, int32_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n / 2) {
+    const int32_t a = perm[i], b = perm[n - 1 - i];
+    perm[i] = b;
+    perm[n - 1 - i] = a;
+  }
+}
+
+// One engine run (variant 0) from `start` over the array's buffers, queue in `a.perm`; waits for it and
+// returns its status.
+int noalloc_run(const NoAllocDevice& D, RunArgs a, int32_t start, bool bfs) {
+  const int32_t n = a.n;
+  const int init_blocks = (int)std::min<int64_t>(2048, ((int64_t)n + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_init, dim3(init_blocks), dim3(256), 0, nullptr, a, start, n + 1);  // every record cleared
+  const int workers = std::max(1, std::min(n > 16384 ? 256 : 64, D.max_resident));
+  void* kern = reorder_kernel(0, false, bfs, false);
+  void* args[] = {&a};
+  RCM_HIP(hipLaunchKernel(kern, dim3(workers), dim3(kVarThreads[0]), args, 0, nullptr));
+  RCM_HIP(hipGetLastError());
+  Control h{};
+  RCM_HIP(hipMemcpy(&h, a.ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  if (h.error == ERR_TIMEOUT) return RCM_ETIMEOUT;
+  if (h.error == ERR_CAPACITY || h.error == ERR_LONGROW) return RCM_ECAPACITY;
+  if (h.error != ERR_NONE || h.nbatches == 0) return RCM_EDEVICE;
+  if ((int32_t)h.placed != n) return RCM_ENOTCONNECTED;
+  return RCM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int rcm_selftest_primitives(int device, const uint32_t* in, uint32_t* out) {
@@ -1822,6 +1905,103 @@ int rcm_batch_autotune(rcm_batch* b, const int32_t* start, int32_t repeats, rcm_
   best.threads = kVarThreads[best.variant];
   b->have_perm = false;
   *out = best;
+  return RCM_OK;
+}
+
+// rcm_gpu_batch on a device-resident CSR with no allocation: every engine buffer is in `scratch` (see above).
+int rcm_gpu_batch_noalloc(int32_t n, int64_t nnz, const int32_t* d_row_ptr, const int32_t* d_col_idx,
+                          int32_t start_node, uint32_t flags, int32_t* d_perm_out, int64_t* bandwidth_out,
+                          int32_t* start_node_out) {
+  if (n < 1 || nnz < 0 || nnz > INT32_MAX || !d_row_ptr || (nnz > 0 && !d_col_idx) || !d_perm_out || start_node >= n)
+    return RCM_EINVAL;
+  if (n > kNoAllocMaxN) return RCM_ECAPACITY;  // beyond the fixed array: use rcm_gpu_batch or a context
+  int dev = 0;
+  RCM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return RCM_EDEVICE;
+  NoAllocDevice& D = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(D.mu);
+  if (!D.base) {  // set-up, once per device, with that device current
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RCM_EDEVICE;
+    int per_cu = 0, cus = 0;
+    RCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reorder_kernel(0, false, false, false),
+                                                         kVarThreads[0], 0));
+    RCM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    D.max_resident = std::max(1, per_cu) * std::max(1, cus);
+    D.base = static_cast<unsigned char*>(p);
+  }
+  const NoAllocLayout L = noalloc_layout((size_t)n);
+  unsigned char* base = D.base;
+  uint32_t* words = reinterpret_cast<uint32_t*>(base + L.words);
+  // validation: the flags and longest row start at zero (the array keeps the previous call's contents)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, nullptr, words, 16, 0u);
+  const int vblocks = (int)std::min<int64_t>(4096, (std::max<int64_t>(nnz, n) + 255) / 256);
+  hipLaunchKernelGGL(k_validate, dim3(std::max(vblocks, 1)), dim3(256), 0, nullptr, d_row_ptr, d_col_idx, n, nnz,
+                     words);
+  RCM_HIP(hipGetLastError());
+  uint32_t vflags[2] = {0, 0};
+  RCM_HIP(hipMemcpy(vflags, words, sizeof(vflags), hipMemcpyDeviceToHost));
+  if (vflags[0] != 0) return RCM_EINVAL;
+
+  RunArgs a{};
+  a.row_ptr = d_row_ptr;
+  a.col_idx = d_col_idx;
+  a.perm = d_perm_out;  // the queue: the start search's sweeps and then the order itself
+  a.qinfo = reinterpret_cast<uint64_t*>(base + L.qinfo);
+  a.claim = reinterpret_cast<uint64_t*>(base + L.claim);
+  a.bstat = reinterpret_cast<uint64_t*>(base + L.bstat);
+  a.brange = reinterpret_cast<uint64_t*>(base + L.brange);
+  a.ctl = reinterpret_cast<Control*>(base + L.ctl);
+  a.n = n;
+  a.bcap = n + 1;
+  a.budget_ticks = 20000ull * 100000ull;  // 20 s watchdog
+  a.sort_mode = 0;
+  a.seg_threshold = kDefaultSegThreshold;
+  a.claim_filter = kDefaultClaimFilter;
+
+  int32_t start = start_node;
+  if (start < 0) {  // SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:92-141)
+    int32_t* level = reinterpret_cast<int32_t*>(base + L.level);
+    start = n / 2;
+    if (n == 1) {
+      start = 0;
+    } else {
+      RunArgs ab = a;
+      ab.level = level;
+      uint32_t lastdist = 0;
+      const int32_t w = std::min<int32_t>(32, n - 1);
+      int32_t nodes[32], dists[32], degs[32];
+      uint64_t rows[32];
+      for (;;) {
+        const int rc = noalloc_run(D, ab, start, true);
+        if (rc) return rc;
+        RCM_HIP(hipMemcpy(nodes, d_perm_out + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost));
+        RCM_HIP(hipMemcpy(dists, level + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost));
+        RCM_HIP(hipMemcpy(rows, ab.qinfo + (n - w), (size_t)w * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < w; ++i) degs[i] = (int32_t)((uint32_t)(rows[i] >> 32) - (uint32_t)rows[i]);
+        if (peripheral_step(nodes, dists, degs, w, &lastdist, &start)) break;
+      }
+    }
+  }
+  int rc = noalloc_run(D, a, start, false);
+  if (rc) return rc;
+  if (bandwidth_out && (flags & RCM_FLAG_BANDWIDTH)) {  // computeBandwidth (src/main.cpp:143-184)
+    int32_t* inv = reinterpret_cast<int32_t*>(a.claim);  // the claims are no longer needed
+    const int blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, nullptr, words + 8, 1, 0u);
+    hipLaunchKernelGGL(k_inverse, dim3(blocks), dim3(256), 0, nullptr, d_perm_out, n, inv);
+    hipLaunchKernelGGL(k_bandwidth, dim3(blocks), dim3(256), 0, nullptr, d_row_ptr, d_col_idx, inv, n, words + 8);
+    RCM_HIP(hipGetLastError());
+    uint32_t bw = 0;
+    RCM_HIP(hipMemcpy(&bw, words + 8, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *bandwidth_out = (int64_t)bw;
+  }
+  if (flags & RCM_FLAG_REVERSE) {
+    hipLaunchKernelGGL(k_reverse_inplace, dim3((n / 2 + 255) / 256 + 1), dim3(256), 0, nullptr, d_perm_out, n);
+    RCM_HIP(hipGetLastError());
+    RCM_HIP(hipStreamSynchronize(nullptr));
+  }
+  if (start_node_out) *start_node_out = start;
   return RCM_OK;
 }
 
