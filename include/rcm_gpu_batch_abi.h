@@ -52,7 +52,7 @@ enum rcm_status {
   RCM_ENOTFOUND = -9,      /* serving: unknown request id */
   RCM_EAGAIN = -10,        /* serving: the request has not finished within the wait's time limit */
   RCM_EIO = -11,           /* a matrix file could not be opened, read or written */
-  RCM_ENOTSYMMETRIC = -12  /* the device connectivity repair needs a structurally symmetric matrix */
+  RCM_ENOTSYMMETRIC = -12  /* reserved (no longer returned: the device repair handles non-symmetric matrices) */
 };
 
 /* One-shot call on host buffers: upload, (optional GPU start search), reorder, (optional bandwidth),
@@ -93,7 +93,8 @@ enum rcm_param {
                                     graphs. The structure of GPU_SIMPLE (src/device/rcm_gpu.cu:134-330) with
                                     the serial order. workers is 0. */
   RCM_PARAM_STRICT = 6,      /* 1 = formal agent-scope release/acquire fences instead of sc1 + s_waitcnt */
-  RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile) */
+  RCM_PARAM_PROFILE = 7,     /* 1 = instrumented reorder kernel (per-phase time sums, rcm_ctx_profile);
+                                2 = the same plus a per-batch trace (rcm_ctx_trace) */
   RCM_PARAM_GRAPH = 8,       /* rcm_batch only: 1 = replay the run's launches from a captured hipGraph */
   RCM_PARAM_CLAIM_FILTER = 9 /* 0 = atomicMin on every adjacency entry; 1 = read the claim word first and skip
                                 entries already claimed by a smaller code (plain load); 2 = same, device-scope */,
@@ -134,6 +135,16 @@ int rcm_ctx_bandwidth(rcm_ctx* ctx, int64_t* bandwidth_out);
  * (src/main.cpp:143-184) for orderings produced elsewhere. The ordering becomes the context's result. */
 int rcm_ctx_bandwidth_of(rcm_ctx* ctx, const int32_t* perm, int64_t* bandwidth_out);
 
+/* Checks on the GPU that an order is exactly the stable Cuthill-McKee order of the loaded matrix from `start`
+ * (RealCPU::rcm_seq<true>, src/rcm_cpu.cpp:57-104), without running the serial algorithm: perm[0] = start, perm
+ * is a bijection, every other node's earliest-placed neighbour precedes it, and perm[1..] increases strictly in
+ * (position of that neighbour, row length, first index in its row) -- conditions that only that order meets.
+ * perm: host, n entries, RCM order when flags has RCM_FLAG_REVERSE; NULL checks the context's last result (then
+ * the flag is ignored). An external perm discards the context's last result. *result_out: 0 when perm is the
+ * order, else 1 + a position where it departs. No single reference counterpart: an O(n + nnz) extension of
+ * checkPermutation (src/main.cpp:186-226). */
+int rcm_ctx_verify(rcm_ctx* ctx, int32_t start, const int32_t* perm, uint32_t flags, int64_t* result_out);
+
 /* Copy the last ordering to the host (n entries), reversed when RCM_FLAG_REVERSE was set in rcm_ctx_run. */
 int rcm_ctx_read_perm(rcm_ctx* ctx, int32_t* perm_out);
 /* Device pointer to the last ordering (n entries, same orientation as rcm_ctx_read_perm). */
@@ -157,6 +168,11 @@ int rcm_ctx_timing(rcm_ctx* ctx, rcm_timing_t* out);
  * for acquire, parents, claim, finalise, sort, cut, prefix, write; then counts of batches, early batches,
  * owned children and adjacency slots. Copies min(count, 16) words. */
 int rcm_ctx_profile(rcm_ctx* ctx, uint64_t* out, int32_t count);
+/* Per-batch trace of the last run with RCM_PARAM_PROFILE = 2: 12 words per batch id (id | kept << 32 | early
+ * << 33; parent range begin | end << 32; children | child batches << 32; first child batch id; 100 MHz
+ * timestamps at entry, claims done, claim completeness known, cut done, prefix known, records published;
+ * two unused words). Copies min(words, 12 * (n + 1)) words; ids the run did not use are zero. */
+int rcm_ctx_trace(rcm_ctx* ctx, uint64_t* out, int64_t words);
 
 /* ---- many matrices, device-resident: one persistent launch reorders all of them ---- */
 /* No single reference counterpart: GPU_BATCH::rcm (src/device/rcm_gpu_batch.cu:42-112) applied to every

@@ -599,6 +599,17 @@ class RcmContext:
         _check(self._lib.rcm_ctx_bandwidth_of(self._h, _p32(p), C.byref(bw)), "bandwidth_of")
         return int(bw.value)
 
+    def verify(self, start: int, perm=None, flags: int = 0) -> int:
+        """0 when perm (host; None: the last result) is exactly the stable CM order of the loaded matrix from
+        start (its reverse with RCM_FLAG_REVERSE), else 1 + a position where it departs. Checked on the GPU."""
+        res = C.c_int64()
+        p = None if perm is None else _i32(perm)
+        if p is not None and p.shape[0] != self.n:
+            return 1
+        _check(self._lib.rcm_ctx_verify(self._h, int(start), None if p is None else _p32(p), flags, C.byref(res)),
+               "verify")
+        return int(res.value)
+
     def perm(self) -> np.ndarray:
         out = np.empty(self.n, np.int32)
         _check(self._lib.rcm_ctx_read_perm(self._h, _p32(out)), "read_perm")
@@ -612,6 +623,7 @@ class RcmContext:
         out = {ph: buf[i] * 0.01 for i, ph in enumerate(PROFILE_PHASES)}  # 100 MHz ticks -> us
         out.update(batches=int(buf[8]), early=int(buf[9]), children=int(buf[10]), slots=int(buf[11]))
         out["compact"] = buf[12] * 0.01  # compaction, before "sort" (the ordering itself)
+        out["kept"] = int(buf[13])  # batches processed by the workgroup that produced them
         return out
 
     def timing(self) -> Timing:

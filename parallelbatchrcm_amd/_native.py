@@ -124,6 +124,8 @@ class TuneResultT(C.Structure):
 _SIGNATURES = {
     # rcm_gpu_batch_abi.h
     "rcm_gpu_batch": (C.c_int, [C.c_int32, C.c_int64, I32P, I32P, C.c_int32, C.c_uint32, I32P, I64P, I32P]),
+    "rcm_ctx_verify": (C.c_int, [C.c_void_p, C.c_int32, I32P, C.c_uint32, I64P]),
+    "rcm_ctx_trace": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_int64]),
     "rcm_gpu_batch_noalloc": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32,
                                         C.c_void_p, I64P, I32P]),
     "rcm_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),

@@ -52,8 +52,15 @@ __device__ __forceinline__ void consume_fence() {
 __device__ __forceinline__ uint64_t atomic_min_u64(uint64_t* p, uint64_t v) {
   return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint32_t atomic_min_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ uint64_t atomic_add_u64(uint64_t* p, uint64_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The same without a result: the compiler issues the atomic without a return, so nothing waits for it.
+__device__ __forceinline__ void atomic_add_u64_nowait(uint64_t* p, uint64_t v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t atomic_add_u32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

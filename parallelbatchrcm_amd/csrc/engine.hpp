@@ -82,6 +82,8 @@ struct RunArgs {
   int32_t sort_mode;      // 0 auto, 1 radix, 2 segment rank
   int32_t seg_threshold;  // auto mode: segment rank when every parent row has at most this many entries
   uint64_t* prof;         // PROF builds: per-phase tick sums (kProfSlots entries)
+  uint64_t* trace;        // PROF builds, nullable: kTraceWords words per batch id below trace_cap
+  int32_t trace_cap;
   uint32_t* jobs_done;    // many-matrix launches: count of finished matrices (nullptr for one matrix)
   int32_t claim_filter;   // 0: atomicMin on every entry; 1: read the claim word first (plain load), 2: same
                           // with a device-scope load; the atomic is issued only if the word is still larger
@@ -95,9 +97,18 @@ __device__ __forceinline__ uint64_t claim_peek(const RunArgs& a, int32_t c) {
 
 // Marks the run finished (complete or failed). The first marker also counts the matrix as finished for a
 // many-matrix launch, so each matrix is counted exactly once.
-__device__ __forceinline__ void mark_done(const RunArgs& a) {
-  const uint32_t was = __hip_atomic_exchange(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (was == 0 && a.jobs_done) atomic_add_u32(a.jobs_done, 1u);
+__device__ __forceinline__ void mark_done(Control* ctl, uint32_t* jobs_done) {
+  const uint32_t was = __hip_atomic_exchange(&ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (was == 0 && jobs_done) atomic_add_u32(jobs_done, 1u);
+}
+__device__ __forceinline__ void mark_done(const RunArgs& a) { mark_done(a.ctl, a.jobs_done); }
+
+// The run is complete when every queue entry is written: the progress word's low half (entries written) equals
+// its high half (the total, set by the batch after which no batch can exist). Batches add to it without waiting
+// for the result; the workers waiting for work test it and mark the run done.
+__device__ __forceinline__ bool run_complete(const Control* ctl) {
+  const uint64_t pr = ld_sync(&ctl->progress);
+  return (pr >> 32) != 0 && (uint32_t)pr == (uint32_t)(pr >> 32);
 }
 
 // Phase profile (PROF builds): 100 MHz ticks summed over all batches, plus event counts.
@@ -115,8 +126,16 @@ enum : int {
   PF_CHILDREN = 10, // sum of owned children
   PF_SLOTS = 11,    // sum of adjacency slots
   PF_COMPACT = 12,  // compaction of owned children (PF_SORT is the ordering after it)
-  kProfSlots = 16
+  PF_KEPT = 13,     // count of batches processed by their producer (no acquire)
+  PF_TRACE0 = 16,   // thread 0's timestamps of the current batch (trace), 5 slots
+  kProfSlots = 16,
+  kProfRegs = 21
 };
+
+// Per-batch trace of PROF builds (RunArgs::trace): batch id | kept << 32 | early << 33, range s | e << 32,
+// children | child batches << 32, first child batch id, then 100 MHz timestamps at entry, claims done, claim
+// completeness known (finalise), cut done, prefix known, records published.
+constexpr int kTraceWords = 12;
 
 // Batch geometry: T threads, at most P parents and S adjacency slots per batch.
 template <int T_, int S_, int P_>
@@ -149,6 +168,8 @@ struct Smem {
   int32_t b, s, e, early, quit;
   int32_t kept;  // the batch in b / s / e is the previous batch's first child, its parents staged in key[kfin]
   int32_t kfin;
+  int32_t kearly;  // the kept batch directly follows its producer: every batch before it has claimed
+  uint32_t wearly[G::W];  // per wave: its look-back window, read before its atomics, showed every claim done
   uint32_t tick;  // the ticket counter, read during the look-back
   uint32_t cnt, nb, excl_off, excl_nb, maxdeg, maxpdeg;
   static_assert(HS >= 2 * G::P, "segment starts and degree bases of every parent fit in hist");
@@ -668,10 +689,22 @@ __device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_
         st_sync(&a.ctl->nbatches, b + 1);
         st_sync(&a.ctl->placed, xo + cnt);
       }
-      const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
-      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) mark_done(a);
+      atomic_add_u64_nowait(&a.ctl->progress, add);  // completion is seen by the workers waiting for work
     }
   }
+}
+
+// One wave's verdict on its look-back window `w` (lane l: the status word of batch b - 1 - l, neutral words
+// beyond the window), written to the wave's LDS flag. Called after the wait that returns the window's loads and
+// before the wave's atomics; the compiler barrier keeps both the loads and the atomics on their side of it, so
+// the window is read before any atomic of the wave is performed.
+template <class SM>
+__device__ __forceinline__ void window_verdict(SM& sm, uint64_t w) {
+  const uint32_t st = st_state(w);
+  const uint64_t incm = ballot(st >= ST_DINC), notm = ballot(st < ST_DLOC);
+  const bool ok = incm != 0 && (notm & ((incm & (~incm + 1)) - 1)) == 0;
+  if (lane_id() == 0) sm.wearly[wave_id()] = ok ? 1u : 0u;
+  __asm__ volatile("" ::: "memory");
 }
 
 // ---- one batch ------------------------------------------------------------------------------------------
@@ -716,7 +749,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     if constexpr (BFS) sm.plevel[tid] = lev;
   }
   if (wave_id() == 0) {
-    const bool ok = !kept && claims_complete_before(a, (int64_t)b);
+    const bool ok = kept ? sm.kearly != 0 : claims_complete_before(a, (int64_t)b);
     if (lane_id() == 0) sm.early = ok ? 1 : 0;
   }
   // Slot -> parent marks for the claims: si[first slot of parent i] = i + 1, 0 on every other slot, and for each
@@ -755,9 +788,8 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
                                               sm.early != 0, deadline);
     return;
   }
-  const bool early = sm.early != 0;
 
-  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PARENTS] += _t - pf[15]; pf[15] = _t; } }
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_TRACE0] = pf[15]; pf[PF_PARENTS] += _t - pf[15]; pf[15] = _t; } }
   // B. claim every adjacency slot (striped: thread t owns slots t, t+T, ..., so round k of wave w is the 64-slot
   //    chunk kW + w). Staged so that all column loads, then all atomics and row-bound loads of a thread are in
   //    flight together. A slot's parent is the prefix maximum of the chunk's parent marks.
@@ -767,6 +799,16 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
   uint16_t ci16[EPT];
   const uint32_t n_chunks = (wave_uniform(D) + kWave - 1) / kWave;
   const uint32_t wave0 = wave_uniform((uint32_t)wave_id());
+  // A second chance for the early path: each wave reads one look-back window of status words in the same round
+  // trip as its claim pre-reads (column loads without the filter), so the read is done before any of its
+  // atomics issue. If every wave's window shows every earlier batch claim-complete, no earlier batch can take
+  // a child from this batch any more, and the batch may publish DINC at once. Every wave reads its window,
+  // those without slots too: for a batch without slots (the empty records a long row leaves) the windows are
+  // all that justifies DINC.
+  const int64_t widx = (int64_t)b - 1 - lane_id();
+  uint64_t wword = lane_id() >= kLookWin ? mk_status(ST_DLOC, 0u, 0u) : mk_status(ST_DINC, 0u, 0u);
+  const bool wload = lane_id() < kLookWin && widx >= 0;
+  if (wload && !a.claim_filter) wword = ld_sync(&a.bstat[widx]);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     code[k] = 0;
@@ -794,6 +836,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     // read the claim words first; atomics only where the word is still larger than our code, and the row bounds
     // (the child's degree) only for those slots, in the same round trip as their atomics: a slot whose pre-read
     // already lost can never own its child
+    if (wload) wword = ld_sync(&a.bstat[widx]);
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       oldv[k] = 0;
@@ -802,6 +845,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       if ((uint32_t)(tid + k * T) < D) oldv[k] = claim_peek(a, cc[k]);
     }
     __builtin_amdgcn_s_waitcnt(0);
+    window_verdict(sm, wword);
 #pragma unroll
     for (int k = 0; k < EPT; ++k)
       if ((uint32_t)(tid + k * T) < D && oldv[k] > code[k]) {
@@ -810,6 +854,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
         cre[k] = (uint32_t)a.row_ptr[cc[k] + 1];
       }
   } else {
+    window_verdict(sm, wword);
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       oldv[k] = 0;
@@ -833,8 +878,14 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     }
   }
   __syncthreads();
+  bool early = sm.early != 0;
+  if (!early) {
+    early = true;
+#pragma unroll
+    for (int w2 = 0; w2 < W; ++w2) early = early && sm.wearly[w2] != 0;
+  }
 
-  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CLAIM] += _t - pf[15]; pf[15] = _t; } }
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_TRACE0 + 1] = _t; pf[PF_CLAIM] += _t - pf[15]; pf[15] = _t; } }
   // won[k]: slot tid + kT owns its child. Decided in registers on the usual path, from the LDS words after the
   // in-batch correction on the early one.
   bool won[EPT];
@@ -876,7 +927,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       if (won[k]) won[k] = ld_sync(&a.claim[cc[k]]) == code[k];
   }
 
-  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_FINAL] += _t - pf[15]; pf[15] = _t; } }
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_TRACE0 + 2] = _t; pf[PF_FINAL] += _t - pf[15]; pf[15] = _t; } }
   // C. compact owned children in slot order. The slots form chunks of 64 (chunk c: slots [64c, 64c + 64)); wave w
   //    holds chunks w, w + W, w + 2W, ... (lane l: slot 64c + l, the claim's striping) with each slot's ownership,
   //    degree, parent and row entry in registers, and skips the chunks at or beyond D. Ballots rank owned slots
@@ -1048,7 +1099,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     }
   }
 
-  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_CUT] += _t - pf[15]; pf[15] = _t; } }
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_TRACE0 + 3] = _t; pf[PF_CUT] += _t - pf[15]; pf[15] = _t; } }
   // F. commit: publish the aggregate, look back for the exclusive prefix, publish the inclusive record. The
   //    ticket counter is read beside the look-back (one round trip for both): it tells whether the first child
   //    batch's ticket has been handed out already.
@@ -1074,7 +1125,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     return;
   }
 
-  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_PREFIX] += _t - pf[15]; pf[15] = _t; } }
+  if constexpr (PROF) { if (tid == 0) { const uint64_t _t = wall_ticks(); pf[PF_TRACE0 + 4] = _t; pf[PF_PREFIX] += _t - pf[15]; pf[15] = _t; } }
   // G. write children (queue entry, row bounds, level), then publish the child batch records.
   //    Work-first: when the first child batch's ticket has already been handed out (so every batch below it
   //    is held by a running workgroup, and waiting on them cannot deadlock), this workgroup keeps that batch
@@ -1113,21 +1164,31 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
         st_sync(&a.ctl->nbatches, b + 1);
         st_sync(&a.ctl->placed, xo + cnt);
       }
-      const uint64_t now = atomic_add_u64(&a.ctl->progress, add) + add;
-      if ((now >> 32) != 0 && (uint32_t)now == (uint32_t)(now >> 32)) mark_done(a);
+      atomic_add_u64_nowait(&a.ctl->progress, add);  // completion is seen by the workers waiting for work
     }
     if (keep) {  // the caller processes this batch next, without acquiring it (read after its barrier)
       sm.kept = 1;
       sm.kfin = fin;
+      sm.kearly = xb == b + 1 ? 1 : 0;  // no batch between this one and the kept one: all before it have claimed
       sm.b = (int32_t)xb;
       sm.s = (int32_t)xo;
       sm.e = (int32_t)(xo + kend);
     }
     if constexpr (PROF) {
       const uint64_t _t = wall_ticks();
+      if (a.trace && (int64_t)b < (int64_t)a.trace_cap) {
+        uint64_t* w = a.trace + (size_t)b * kTraceWords;
+        w[0] = (uint64_t)b | ((uint64_t)(kept ? 1 : 0) << 32) | ((uint64_t)(early ? 1 : 0) << 33);
+        w[1] = (uint64_t)(uint32_t)s | ((uint64_t)(uint32_t)e << 32);
+        w[2] = (uint64_t)cnt | ((uint64_t)nb << 32);
+        w[3] = xb;
+        for (int q = 0; q < 5; ++q) w[4 + q] = pf[PF_TRACE0 + q];
+        w[9] = _t;
+      }
       pf[PF_WRITE] += _t - pf[15];
       pf[15] = _t;
       pf[PF_BATCHES] += 1;
+      pf[PF_KEPT] += keep ? 1 : 0;
       pf[PF_EARLY] += early ? 1 : 0;
       pf[PF_CHILDREN] += cnt;
       pf[PF_SLOTS] += D;
@@ -1142,9 +1203,9 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   RCM_LDS(SmemT, sm);
   const int tid = threadIdx.x;
   const uint64_t deadline = wall_ticks() + a.budget_ticks;
-  uint64_t pf[kProfSlots];  // thread 0's phase sums (PROF builds)
+  uint64_t pf[kProfRegs];  // thread 0's phase sums and trace timestamps (PROF builds)
   if constexpr (PROF) {
-    for (int i = 0; i < kProfSlots; ++i) pf[i] = 0;
+    for (int i = 0; i < kProfRegs; ++i) pf[i] = 0;
   }
   if (tid == 0) sm.kept = 0;
   __syncthreads();
@@ -1164,6 +1225,7 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
             }
             if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done word rarely (measured faster)
               if (ld_sync(&a.ctl->done)) { quit = 1; break; }
+              if (run_complete(a.ctl)) { mark_done(a); quit = 1; break; }
               if (wall_ticks() > deadline) { raise_error(a, ERR_TIMEOUT); quit = 1; break; }
             }
             __builtin_amdgcn_s_sleep(1);
@@ -1196,7 +1258,7 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   }
   if constexpr (PROF) {
     if (tid == 0 && a.prof)
-      for (int i = 0; i <= PF_COMPACT; ++i) atomic_add_u64(&a.prof[i], pf[i]);
+      for (int i = 0; i <= PF_KEPT; ++i) atomic_add_u64(&a.prof[i], pf[i]);
   }
 }
 

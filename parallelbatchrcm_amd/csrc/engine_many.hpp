@@ -142,6 +142,10 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
         }
         if ((spin + 1) % kQuitPoll == 0) {  // idle workers poll the done and abort words rarely
           if (wave_uniform(ld_sync(&J.ctl->done))) break;  // no batch t: the job has finished
+          if (wave_uniform(run_complete(J.ctl) ? 1u : 0u)) {  // every entry written: mark the job done
+            if (lane == 0) mark_done(J.ctl, &m.g->jobs_done);
+            break;
+          }
           if (wave_uniform(wall_ticks() > deadline ? 1u : 0u)) break;
           if (wave_uniform(ld_sync(&m.g->abort))) break;
         }

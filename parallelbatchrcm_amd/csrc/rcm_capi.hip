@@ -23,6 +23,7 @@
 #include "hip_exec.hpp"
 #include "level_cm.hpp"
 #include "rcm_gpu_batch_abi.h"
+#include "verify.hpp"
 
 using namespace rcm;
 
@@ -33,6 +34,8 @@ constexpr int64_t kDefaultHelperNodes = 8192;  // a matrix of n rows: at most n 
 // Serving lanes give each request more helpers: latency, not throughput, is the aim there. Measured on MI355X
 // (Poisson stream, 95% mixed sizes and 5% R-MAT): 2048 against 8192 halves p50 at 200 and 400 requests/s.
 constexpr int64_t kServerHelperNodes = 2048;
+// Serving groups requests of at least this many rows only with each other.
+constexpr int32_t kServeLargeN = 1 << 18;
 
 static_assert(sizeof(rcm_tune_result) == 40, "the Python mirror (TuneResultT) assumes this layout");
 
@@ -148,6 +151,8 @@ struct rcm_ctx {
   int64_t profile = 0;
   int64_t batch_cap = 0;     // RCM_PARAM_BATCH_CAP: batch records a run may use (0 = n + 1, all it can need)
   uint64_t* prof = nullptr;  // kProfSlots words
+  uint64_t* trace = nullptr;  // RCM_PARAM_PROFILE = 2: kTraceWords words per batch id
+  int64_t trace_cap = 0;
   // matrix
   int32_t n = 0;
   int64_t nnz = 0;
@@ -272,6 +277,17 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   const bool prof = c->profile && !bfs && !c->strict;
   a.prof = prof ? c->prof : nullptr;
   if (prof) RCM_HIP(hipMemsetAsync(c->prof, 0, kProfSlots * sizeof(uint64_t), c->stream));
+  if (prof && c->profile == 2) {  // per-batch trace
+    if (c->trace_cap < (int64_t)c->n + 1) {
+      dev_free(c->trace);
+      c->trace_cap = 0;
+      if (dev_alloc(&c->trace, ((size_t)c->n + 1) * kTraceWords) != hipSuccess) return RCM_ENOMEM;
+      c->trace_cap = (int64_t)c->n + 1;
+    }
+    RCM_HIP(hipMemsetAsync(c->trace, 0, (size_t)c->trace_cap * kTraceWords * sizeof(uint64_t), c->stream));
+    a.trace = c->trace;
+    a.trace_cap = (int32_t)c->trace_cap;
+  }
   // clear every record an earlier run may have written, also beyond this matrix's n (a previous, larger
   // matrix in the same buffers): workers read records ahead of their producers
   const int32_t bclear = std::max(1, std::min(c->dirty, c->cap + 1));
@@ -330,6 +346,35 @@ bool peripheral_step(const int32_t* nodes, const int32_t* dists, const int32_t* 
   }
   *lastdist = (uint32_t)dists[w - 1];
   return false;
+}
+
+// The order check of verify.hpp over the context's matrix for the device order d_perm (n entries). Uses the
+// inverse and claim buffers as scratch. *result_out: 0 when d_perm is the order, else 1 + a position where it
+// departs.
+int verify_device(rcm_ctx* c, const int32_t* d_perm, int32_t start, bool bfs, int64_t* result_out) {
+  const int32_t n = c->n;
+  int32_t* pos = c->inv;
+  uint32_t* ppos = reinterpret_cast<uint32_t*>(c->claim);  // claim holds 2n words
+  uint32_t* rix = ppos + n;
+  uint32_t* bad = c->scratch + 9;
+  const unsigned g1 = (unsigned)((n + 255) / 256), gw = (unsigned)(((int64_t)n * kWave + 255) / 256);
+  hipLaunchKernelGGL(k_vfy_fill, dim3(g1), dim3(256), 0, c->stream, reinterpret_cast<uint32_t*>(pos), (int64_t)n,
+                     0xFFFFFFFFu);
+  hipLaunchKernelGGL(k_vfy_fill, dim3(2 * g1), dim3(256), 0, c->stream, ppos, 2 * (int64_t)n, kVerifyNone);
+  hipLaunchKernelGGL(k_vfy_fill, dim3(1), dim3(64), 0, c->stream, bad, (int64_t)1, kVerifyNone);
+  hipLaunchKernelGGL(k_vfy_positions, dim3(g1), dim3(256), 0, c->stream, d_perm, n, start, pos, bad);
+  hipLaunchKernelGGL(k_vfy_parents, dim3(gw), dim3(256), 0, c->stream, (const int32_t*)c->rp, (const int32_t*)c->ci,
+                     n, (const int32_t*)pos, ppos);
+  hipLaunchKernelGGL(k_vfy_rowindex, dim3(gw), dim3(256), 0, c->stream, (const int32_t*)c->rp,
+                     (const int32_t*)c->ci, n, (const int32_t*)pos, (const uint32_t*)ppos, rix);
+  hipLaunchKernelGGL(k_vfy_keys, dim3(g1), dim3(256), 0, c->stream, (const int32_t*)c->rp, d_perm, n,
+                     (const uint32_t*)ppos, (const uint32_t*)rix, bfs ? 1 : 0, bad);
+  RCM_HIP(hipGetLastError());
+  uint32_t h = 0;
+  RCM_HIP(hipMemcpyAsync(&h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  RCM_HIP(hipStreamSynchronize(c->stream));
+  *result_out = h == kVerifyNone ? 0 : (int64_t)h + 1;
+  return RCM_OK;
 }
 
 int load_common(rcm_ctx* c, int32_t n, int64_t nnz) {
@@ -728,13 +773,19 @@ void lane_loop(rcm_server* s, int lane) {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
       if (s->pending.empty()) return;  // stopping, nothing left
-      // a group: first come first served, looking at most 8 requests ahead for ones that fit the lane
+      // a group: first come first served, looking at most 8 requests ahead for ones that fit the lane and are
+      // of the first request's size class (requests of kServeLargeN rows or more are grouped only with each
+      // other, so small requests do not wait behind them); at most a lanes-th of what is waiting, so that a
+      // burst spreads over every lane
+      const bool large = s->pending.front()->n >= kServeLargeN;
+      const size_t share = (s->pending.size() + (size_t)s->lanes - 1) / (size_t)s->lanes;
+      const size_t cap = std::min<size_t>((size_t)s->cfg.slots, std::max<size_t>(1, share));
       GroupSize gs;
       size_t skipped = 0;
-      for (size_t i = 0; i < s->pending.size() && skipped < 8 && (int32_t)grp.size() < s->cfg.slots;) {
+      for (size_t i = 0; i < s->pending.size() && skipped < 8 && grp.size() < cap;) {
         const auto& jp = s->pending[i];
         const GroupSize next = gs.plus(jp->n, jp->nnz);
-        if (next.bytes() <= s->lane_bytes) {
+        if ((jp->n >= kServeLargeN) == large && next.bytes() <= s->lane_bytes) {
           gs = next;
           grp.push_back(jp);
           s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
@@ -956,6 +1007,7 @@ int rcm_ctx_destroy(rcm_ctx* c) {
   dev_free(c->ctl);
   dev_free(c->scratch);
   dev_free(c->prof);
+  dev_free(c->trace);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->h_word) (void)hipHostFree(c->h_word);
   c->lscratch.release();
@@ -977,7 +1029,7 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
       return v == (c->variant == kLevelVariant ? kLevelDigit : kVarDigit[c->variant]) ? RCM_OK : RCM_EINVAL;
     case RCM_PARAM_VARIANT: if (v < 0 || v > kLevelVariant) return RCM_EINVAL; c->variant = v; return RCM_OK;
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
-    case RCM_PARAM_PROFILE: c->profile = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_PROFILE: if (v < 0 || v > 2) return RCM_EINVAL; c->profile = v; return RCM_OK;
     case RCM_PARAM_BATCH_CAP: if (v < 0) return RCM_EINVAL; c->batch_cap = v; return RCM_OK;
     default: return RCM_EINVAL;
   }
@@ -988,6 +1040,15 @@ int rcm_ctx_profile(rcm_ctx* c, uint64_t* out, int32_t count) {
   RCM_HIP(hipSetDevice(c->device));
   RCM_HIP(hipMemcpy(out, c->prof, (size_t)std::min<int32_t>(count, kProfSlots) * sizeof(uint64_t),
                     hipMemcpyDeviceToHost));
+  return RCM_OK;
+}
+
+int rcm_ctx_trace(rcm_ctx* c, uint64_t* out, int64_t words) {
+  if (!c || !out || words <= 0) return RCM_EINVAL;
+  if (!c->trace) return RCM_ENOTLOADED;
+  RCM_HIP(hipSetDevice(c->device));
+  const int64_t w = std::min<int64_t>(words, c->trace_cap * kTraceWords);
+  RCM_HIP(hipMemcpy(out, c->trace, (size_t)w * sizeof(uint64_t), hipMemcpyDeviceToHost));
   return RCM_OK;
 }
 
@@ -1143,6 +1204,23 @@ int rcm_ctx_bandwidth_of(rcm_ctx* c, const int32_t* perm, int64_t* bw_out) {
   c->have_perm = true;
   c->reversed = false;
   return rcm_ctx_bandwidth(c, bw_out);
+}
+
+int rcm_ctx_verify(rcm_ctx* c, int32_t start, const int32_t* perm, uint32_t flags, int64_t* result_out) {
+  if (!c || !result_out) return RCM_EINVAL;
+  if (!c->rp || c->n == 0) return RCM_ENOTLOADED;
+  if (start < 0 || start >= c->n) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(c->device));
+  const int32_t n = c->n;
+  if (!perm) {  // the context's last order: its CM form is in perm
+    if (!c->have_perm) return RCM_ENOTLOADED;
+    return verify_device(c, c->perm, start, false, result_out);
+  }
+  c->have_perm = false;  // the external order takes the buffer of the reversed result
+  RCM_HIP(hipMemcpyAsync(c->out, perm, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  if (flags & RCM_FLAG_REVERSE)
+    hipLaunchKernelGGL(k_vfy_reverse, dim3((unsigned)((n / 2 + 255) / 256 + 1)), dim3(256), 0, c->stream, c->out, n);
+  return verify_device(c, c->out, start, false, result_out);
 }
 
 int rcm_ctx_read_perm(rcm_ctx* c, int32_t* perm_out) {

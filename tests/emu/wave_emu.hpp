@@ -298,6 +298,12 @@ inline unsigned atomicMax(unsigned* p, unsigned v) {
   }
   return cur;
 }
+inline unsigned atomicMin(unsigned* p, unsigned v) {
+  unsigned cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return cur;
+}
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 inline int __clz(unsigned x) { return x ? __builtin_clz(x) : 32; }
