@@ -46,10 +46,11 @@ int rcm_ops_permute(rcm_ops* ops, int32_t n, int64_t nnz, const int32_t* row_ptr
                     double* values_out);
 
 /* convert + prepareRCM(check_sym) of a COO on device buffers. The result stays in the handle until its next
- * call; read it with rcm_ops_result. RCM_EINVAL: an index out of range, a non-square result, or an empty row
- * (where the reference's repair loop does not terminate); RCM_ENOTSYMMETRIC: the connectivity step needs a
- * structurally symmetric matrix (always the case after the symmetrisation unless a pair is stored twice in one
- * direction only) -- rcm_prepare (include/rcm_formats.h) handles that case on the host. */
+ * call; read it with rcm_ops_result. A structurally symmetric result is repaired by components; any other (no
+ * symmetrisation asked, or a pair stored twice in one direction only) by following its stored rows, as the
+ * reference's walk does. RCM_EINVAL: an index out of range, a non-square result, or a matrix on which the
+ * reference's repair loop does not terminate (an empty row nothing reaches, or a node that no smaller node
+ * reaches and that lies on no cycle). */
 int rcm_ops_canonicalize_device(rcm_ops* ops, int32_t rows, int32_t cols, int64_t nnz, const int32_t* d_row_ids,
                                 const int32_t* d_col_ids, const void* d_values, int32_t value_bytes,
                                 int32_t check_sym, int64_t* links_out);

@@ -386,6 +386,66 @@ __global__ __launch_bounds__(kT) void k_row_starts(const uint64_t* keys, int64_t
     for (int64_t x = r + 1; x <= rows; ++x) rp[x] = (int32_t)nnz;
 }
 
+// Rows longer than kShortRow of the row-sorted entries: long_cnt[r] = 1 and long_len[r] = the length for them,
+// 0 otherwise (entry n of both is 0, for the scans).
+__global__ __launch_bounds__(kT) void k_long_rows(const int32_t* rp, int32_t n, uint32_t* long_len,
+                                                 uint32_t* long_cnt) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r > n) return;
+  const int32_t l = r < n ? rp[r + 1] - rp[r] : 0;
+  long_cnt[r] = l > kShortRow ? 1u : 0u;
+  long_len[r] = l > kShortRow ? (uint32_t)l : 0u;
+}
+
+// Each row of at most kShortRow entries sorted by column, stably (insertion sort; the entries of a row are in
+// input order after the row sort, so equal columns keep it).
+__global__ __launch_bounds__(kT) void k_sort_short_rows(const int32_t* rp, int32_t n, uint64_t* keys,
+                                                       uint32_t* idx) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r >= n) return;
+  const int32_t b = rp[r], e = rp[r + 1];
+  if (e - b > kShortRow) return;
+  for (int32_t i = b + 1; i < e; ++i) {
+    const uint64_t k = keys[i];
+    const uint32_t v = idx[i];
+    int32_t j = i;
+    while (j > b && (uint32_t)keys[j - 1] > (uint32_t)k) {
+      keys[j] = keys[j - 1];
+      idx[j] = idx[j - 1];
+      --j;
+    }
+    keys[j] = k;
+    idx[j] = v;
+  }
+}
+
+// One workgroup per long row: gkey = (ordinal << 32) | column, gval = the entry's input index.
+__global__ __launch_bounds__(kT) void k_coo_long_gather(const uint64_t* keys, const uint32_t* idx, const int32_t* rp,
+                                                       const int32_t* list, const uint32_t* long_off,
+                                                       uint64_t* gkey, uint32_t* gval) {
+  const uint32_t o = blockIdx.x;
+  const int32_t r = list[o];
+  const int32_t b = rp[r], l = rp[r + 1] - b;
+  const uint32_t out = long_off[r];
+  for (int32_t j = threadIdx.x; j < l; j += kT) {
+    gkey[out + (uint32_t)j] = ((uint64_t)o << 32) | (uint32_t)keys[b + j];
+    gval[out + (uint32_t)j] = idx[b + j];
+  }
+}
+
+// The sorted long-row entries back into their rows.
+__global__ __launch_bounds__(kT) void k_coo_long_scatter(const uint64_t* gkey, const uint32_t* gval, int64_t m,
+                                                        const int32_t* list, const uint32_t* long_off,
+                                                        const int32_t* rp, uint64_t* keys, uint32_t* idx) {
+  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (j >= m) return;
+  const uint64_t k = gkey[j];
+  const int32_t r = list[(uint32_t)(k >> 32)];
+  const int64_t dst = (int64_t)rp[r] + (j - (int64_t)long_off[r]);
+  keys[dst] = ((uint64_t)(uint32_t)r << 32) | (uint32_t)k;
+  idx[dst] = gval[j];
+}
+
 template <class V>
 __global__ __launch_bounds__(kT) void k_csr_from_sorted(const uint64_t* keys, const uint32_t* idx, int64_t nnz,
                                                        const V* vin, int32_t* ci_out, V* vout) {
@@ -395,49 +455,45 @@ __global__ __launch_bounds__(kT) void k_csr_from_sorted(const uint64_t* keys, co
   if (vin) vout[q] = vin[idx[q]];
 }
 
-// off-diagonal flags of the CSR's entries (rows from the sorted keys)
-__global__ __launch_bounds__(kT) void k_offdiag_flags(const uint64_t* keys, int64_t nnz, uint32_t* flags) {
-  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (q >= nnz) return;
-  const uint64_t key = keys[q];
-  flags[q] = (uint32_t)(key >> 32) != (uint32_t)key ? 1u : 0u;
-}
-
-// unordered pair key (min << 32 | max) of each off-diagonal entry, compacted by the scanned flags
-__global__ __launch_bounds__(kT) void k_pair_keys(const uint64_t* keys, int64_t nnz, const uint32_t* pos,
-                                                 uint64_t* pk, uint32_t* pidx) {
-  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (q >= nnz) return;
-  const uint64_t key = keys[q];
-  const uint32_t r = (uint32_t)(key >> 32), c = (uint32_t)key;
-  if (r == c) return;
-  const uint32_t lo = r < c ? r : c, hi = r < c ? c : r;
-  pk[pos[q]] = ((uint64_t)lo << 32) | hi;
-  pidx[pos[q]] = (uint32_t)q;
-}
-
-// odd[entry] = 1 for every entry of a pair stored an odd number of times (runs of the sorted pair keys).
-// An odd pair becomes symmetric (each entry gets its transpose); an even pair stays as stored, so the result
-// is structurally symmetric exactly when every even pair is stored as often in each direction: flag[0] = 1
-// where one is not.
-__global__ __launch_bounds__(kT) void k_pair_odd(const uint64_t* pk, const uint32_t* pidx, const uint64_t* keys,
-                                                int64_t m, uint32_t* odd, uint32_t* flag) {
-  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (s >= m) return;
-  if (s > 0 && pk[s - 1] == pk[s]) return;  // not the start of a run
-  int64_t e = s + 1;
-  int64_t upper = 0;  // entries stored above the diagonal
-  for (;;) {
-    const uint64_t key = keys[pidx[e - 1]];
-    upper += (uint32_t)(key >> 32) < (uint32_t)key ? 1 : 0;
-    if (e >= m || pk[e] != pk[s]) break;
-    ++e;
-  }
-  if (((e - s) & 1) == 0) {
-    if (2 * upper != e - s) *flag = 1u;
+// Symmetrisation (include/Graph.h:77-157): an unordered pair stored an odd number of times becomes symmetric
+// (each of its entries gets its transpose), an even pair stays as stored. Decided per entry of the row-sorted
+// CSR: entry (r, c) of a pair stored m_rc times as (r, c) (its run in row r) and m_cr times as (c, r) (a binary
+// search in row c) is odd when m_rc + m_cr is; an even pair with m_rc != m_cr leaves the result not structurally
+// symmetric: flag[0] = 1.
+__global__ __launch_bounds__(kT) void k_pair_parity(const uint64_t* keys, const int32_t* rp, const int32_t* ci,
+                                                   int32_t rows, int64_t nnz, uint32_t* odd, uint32_t* flag) {
+  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (k >= nnz) return;
+  const uint64_t key = keys[k];
+  const int32_t r = (int32_t)(key >> 32), c = (int32_t)(uint32_t)key;
+  if (r == c) {
+    odd[k] = 0u;
     return;
   }
-  for (int64_t x = s; x < e; ++x) odd[pidx[x]] = 1u;
+  int64_t lo = k, hi = k + 1;
+  const int64_t rb = rp[r], re = rp[r + 1];
+  while (lo > rb && ci[lo - 1] == c) --lo;
+  while (hi < re && ci[hi] == c) ++hi;
+  const int64_t m_rc = hi - lo;
+  int64_t m_cr = 0;
+  if (c < rows) {
+    int64_t a = rp[c], b = rp[c + 1];  // first entry >= r
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      if (ci[mid] < r) a = mid + 1;
+      else b = mid;
+    }
+    int64_t e = a, f = rp[c + 1];  // first entry > r
+    while (e < f) {
+      const int64_t mid = (e + f) >> 1;
+      if (ci[mid] <= r) e = mid + 1;
+      else f = mid;
+    }
+    m_cr = e - a;
+  }
+  const bool is_odd = ((m_rc + m_cr) & 1) != 0;
+  odd[k] = is_odd ? 1u : 0u;
+  if (!is_odd && m_rc != m_cr) *flag = 1u;
 }
 
 // flag[0] = 1 for an empty row
@@ -489,13 +545,10 @@ __global__ __launch_bounds__(kT) void k_struct_checks(const int32_t* rp, const i
   }
 }
 
-// parent[v] = the smallest neighbour of v if it is below v (sorted rows: the first column), else v. Each such
-// parent is an edge of the graph, so the forest starts partly joined.
-__global__ __launch_bounds__(kT) void k_cc_init(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
+// parent[v] = v
+__global__ __launch_bounds__(kT) void k_cc_self(int32_t n, uint32_t* parent) {
   const int64_t v = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (v >= n) return;
-  const int32_t a = rp[v];
-  parent[v] = (a < rp[v + 1] && ci[a] < (int32_t)v) ? (uint32_t)ci[a] : (uint32_t)v;
+  if (v < n) parent[v] = (uint32_t)v;
 }
 
 // Root of x with path halving. Parents only ever move to smaller indices, so every chain ends.
@@ -524,27 +577,42 @@ __device__ __forceinline__ uint32_t cc_find_cached(uint32_t* parent, uint32_t x)
   return x;
 }
 
-// Concurrent union-find over the edges (u, v), v < u: the larger root is hooked under the smaller one with a
-// device-coherent compare-and-swap, so each tree's root is its smallest node. Walks use cached loads; a CAS
-// only succeeds on a true root, and when it fails the walk restarts from the parent it returned, so the
-// larger of the two candidates strictly decreases and the loop ends.
-__global__ __launch_bounds__(kT) void k_cc_hook(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
+// Concurrent union-find: the trees of u and v are hooked, the larger root under the smaller one, with a
+// device-coherent compare-and-swap, so each tree's root is its smallest node. Walks use cached loads; a CAS only
+// succeeds on a true root, and when it fails the walk restarts from the parent it returned, so the larger of the
+// two candidates strictly decreases and the loop ends.
+__device__ __forceinline__ void cc_link(uint32_t* parent, uint32_t u, uint32_t v) {
+  uint32_t a = cc_find_cached(parent, u), b = cc_find_cached(parent, v);
+  while (a != b) {
+    const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+    uint32_t expected = hi;
+    if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      break;
+    a = cc_find_cached(parent, expected);
+    b = cc_find_cached(parent, lo);
+  }
+}
+
+// Components by sampled linking (the Afforest scheme of Sutton, Ben-Nun and Barak, IPDPS 2018): link every node
+// to the k-th entry of its row, for the first few k; then only the rows of nodes outside one large tree are linked
+// in full. Rows are structurally symmetric, so an edge with an endpoint outside that tree is linked from that
+// endpoint's row, and edges inside the tree change nothing.
+__global__ __launch_bounds__(kT) void k_cc_link_kth(const int32_t* rp, const int32_t* ci, int32_t n, int32_t k,
+                                                   uint32_t* parent) {
   const int64_t u = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (u >= n) return;
-  for (int32_t k = rp[u]; k < rp[u + 1]; ++k) {
-    const uint32_t v = (uint32_t)ci[k];
-    if (v >= (uint32_t)u) break;  // sorted columns: each undirected edge once
-    uint32_t a = cc_find_cached(parent, (uint32_t)u), b = cc_find_cached(parent, v);
-    while (a != b) {
-      const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
-      uint32_t expected = hi;
-      if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT))
-        break;
-      a = cc_find_cached(parent, expected);  // expected: hi's current parent, below hi
-      b = cc_find_cached(parent, lo);
-    }
-  }
+  const int32_t a = rp[u] + k;
+  if (a < rp[u + 1]) cc_link(parent, (uint32_t)u, (uint32_t)ci[a]);
+}
+
+// Every row entry of each node whose tree is not `giant`, from its first unlinked entry on.
+__global__ __launch_bounds__(kT) void k_cc_link_rest(const int32_t* rp, const int32_t* ci, int32_t n, int32_t from,
+                                                    uint32_t giant, uint32_t* parent) {
+  const int64_t u = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (u >= n) return;
+  if (cc_find_cached(parent, (uint32_t)u) == giant) return;
+  for (int32_t k = rp[u] + from; k < rp[u + 1]; ++k) cc_link(parent, (uint32_t)u, (uint32_t)ci[k]);
 }
 
 __global__ __launch_bounds__(kT) void k_cc_finish(uint32_t* parent, int32_t n) {
@@ -596,6 +664,55 @@ __global__ __launch_bounds__(kT) void k_link_insert(const int32_t* rp, const int
     if (vals) vals2[o] = (V)1;
     ++o;
   }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Connectivity repair of a matrix that is not structurally symmetric (include/Graph.h:160-263).
+// The reference marks the first non-empty row m and everything reachable from it along stored entries
+// (row -> column); then, while a node is unmarked, it takes the lowest one, i, links (i - 1, i) (every node
+// below i is marked), pushes i without marking it and marks what the push reaches. The marked set is always
+// the closure of [0, i) and m under the entries, so with anc(x) = the smallest node with a path of length >= 1
+// to x:
+//   x > m is linked  <=>  no node below x reaches x  <=>  anc(x) >= x;
+// a linked x is marked by its own push only when it lies on a cycle (anc(x) == x). Otherwise the reference
+// links it again forever, and so it does for a node below m (an empty row) that m does not reach: both are
+// the non-terminating case. anc is the fixed point of anc(c) = min over entries (p, c) of min(p, anc(p)),
+// reached by passes over the entries until one changes nothing.
+// ------------------------------------------------------------------------------------------------------
+// *out = the first non-empty row (out starts at n)
+__global__ __launch_bounds__(kT) void k_first_nonempty(const int32_t* rp, int32_t n, uint32_t* out) {
+  const int64_t x = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (x < n && rp[x + 1] > rp[x]) atomicMin(out, (uint32_t)x);
+}
+
+// One pass: every entry (p, c) lowers anc(c) to min(p, anc(p)); *changed = 1 when any value fell.
+__global__ __launch_bounds__(kT) void k_anc_pass(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* anc,
+                                                 uint32_t* changed) {
+  const int64_t p = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t v = min((uint32_t)p, ld_u32(anc + p));  // values only fall: a stale read is an upper bound
+  bool fell = false;
+  for (int32_t k = rp[p]; k < rp[p + 1]; ++k) {
+    uint32_t* t = anc + ci[k];
+    if (ld_u32(t) > v && rcm::atomic_min_u32(t, v) > v) fell = true;
+  }
+  if (fell) st_u32(changed, 1u);
+}
+
+// root[x] = x for a linked node, ~0 otherwise; flag[0] = 1 for the non-terminating case.
+__global__ __launch_bounds__(kT) void k_dir_roots(const uint32_t* anc, int32_t n, uint32_t m, uint32_t* root,
+                                                  uint32_t* flag) {
+  const int64_t xi = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (xi >= n) return;
+  const uint32_t x = (uint32_t)xi, a = anc[x];
+  uint32_t r = ~0u;
+  if (x < m) {
+    if (a != m) flag[0] = 1u;  // an empty row m does not reach
+  } else if (x > m) {
+    if (a == x) r = x;             // linked, marked by its own push (a cycle)
+    else if (a > x) flag[0] = 1u;  // linked, never marked
+  }
+  root[x] = r;
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -735,10 +852,40 @@ int coo_to_csr(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, 
   x.run(k_fill_u32, 1, flag, (int64_t)1, 0u);
   if (nnz > 0) x.run(k_coo_keys, groups(nnz, kT), r, c, nnz, rows, cols, keys, idx, flag);
   if (x.read_u32(flag)) return kOpsInvalid;
-  radix_sort(x, keys, idx, keys2, idx2, nnz, bits_of(cols > 0 ? (uint64_t)(cols - 1) : 0),
-             bits_of(rows > 0 ? (uint64_t)(rows - 1) : 0));
+  // rows first: a stable sort on the row bits alone keeps each row's entries in input order; then every row
+  // by column, stably -- short rows by one thread each, long rows by a radix sort over (long-row ordinal,
+  // column). Fewer passes over all entries than a sort on both words.
+  radix_sort(x, keys, idx, keys2, idx2, nnz, 0, bits_of(rows > 0 ? (uint64_t)(rows - 1) : 0));
   if (nnz > 0) {
     x.run(k_row_starts, groups(nnz, kT), (const uint64_t*)keys, nnz, rows, out.rp);
+    {
+      uint32_t* long_len = x.template alloc<uint32_t>((size_t)rows + 1);
+      uint32_t* long_cnt = x.template alloc<uint32_t>((size_t)rows + 1);
+      uint32_t* tot = x.template alloc<uint32_t>(2);
+      if (x.failed()) return kOpsDevice;
+      x.run(k_long_rows, groups((int64_t)rows + 1, kT), (const int32_t*)out.rp, rows, long_len, long_cnt);
+      scan_u32(x, long_len, long_len, (int64_t)rows + 1, tot);
+      scan_u32(x, long_cnt, long_cnt, (int64_t)rows + 1, tot + 1);
+      x.run(k_sort_short_rows, groups(rows, kT), (const int32_t*)out.rp, rows, keys, idx);
+      const uint32_t long_entries = x.read_u32(tot), long_rows = x.read_u32(tot + 1);
+      if (x.failed()) return kOpsDevice;
+      if (long_rows > 0) {
+        int32_t* list = x.template alloc<int32_t>(long_rows);
+        uint64_t* gk = x.template alloc<uint64_t>(long_entries);
+        uint64_t* gk2 = x.template alloc<uint64_t>(long_entries);
+        uint32_t* gv = x.template alloc<uint32_t>(long_entries);
+        uint32_t* gv2 = x.template alloc<uint32_t>(long_entries);
+        if (x.failed()) return kOpsDevice;
+        x.run(k_long_list, groups(rows, kT), (const int32_t*)out.rp, (const uint32_t*)long_cnt, rows, list);
+        x.run(k_coo_long_gather, (int64_t)long_rows, (const uint64_t*)keys, (const uint32_t*)idx,
+              (const int32_t*)out.rp, (const int32_t*)list, (const uint32_t*)long_len, gk, gv);
+        radix_sort(x, gk, gv, gk2, gv2, (int64_t)long_entries, bits_of(cols > 0 ? (uint64_t)(cols - 1) : 0),
+                   bits_of((uint64_t)long_rows - 1));
+        x.run(k_coo_long_scatter, groups(long_entries, kT), (const uint64_t*)gk, (const uint32_t*)gv,
+              (int64_t)long_entries, (const int32_t*)list, (const uint32_t*)long_len, (const int32_t*)out.rp, keys,
+              idx);
+      }
+    }
     x.run(k_csr_from_sorted<V>, groups(nnz, kT), (const uint64_t*)keys, (const uint32_t*)idx, nnz, vals, out.ci,
           out.vals);
   } else {
@@ -748,9 +895,48 @@ int coo_to_csr(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, 
   return x.failed() ? kOpsDevice : kOpsOk;
 }
 
-// convert + prepareRCM(check_sym) (src/CSR.cpp:170-208, include/Graph.h:74-264) for structurally symmetric
-// results: the connectivity step needs (r, c) <=> (c, r) and no empty row (which the reference's repair loop
-// never leaves). *links_out: links added.
+// Components of a structurally symmetric matrix: root[v] = the smallest node of v's component. The rows of the
+// nodes outside node 0's tree after the sampled rounds are linked in full (for a connected mesh that tree holds
+// nearly every node by then; for any matrix the result is the same, only the work differs).
+template <class X>
+int components(X& x, int32_t n, const int32_t* rp, const int32_t* ci, uint32_t* root) {
+  constexpr int32_t kLinkRounds = 2;
+  x.run(k_cc_self, groups(n, kT), n, root);
+  for (int32_t k = 0; k < kLinkRounds; ++k) x.run(k_cc_link_kth, groups(n, kT), rp, ci, n, k, root);
+  x.run(k_cc_finish, groups(n, kT), root, n);  // flat trees: the test below is one load for most nodes
+  // node 0 is the root of its tree (roots are tree minima): its tree is the one whose rows are skipped
+  x.run(k_cc_link_rest, groups(n, kT), rp, ci, n, kLinkRounds, 0u, root);
+  x.run(k_cc_finish, groups(n, kT), root, n);
+  return x.failed() ? kOpsDevice : kOpsOk;
+}
+
+// The linked nodes of a matrix that is not structurally symmetric: root[x] = x for each, ~0 elsewhere.
+template <class X>
+int directed_roots(X& x, int32_t n, const int32_t* rp, const int32_t* ci, uint32_t* root) {
+  uint32_t* w = x.template alloc<uint32_t>(3);  // first non-empty row, changed, non-terminating
+  uint32_t* anc = x.template alloc<uint32_t>((size_t)n);
+  if (x.failed()) return kOpsDevice;
+  x.run(k_fill_u32, 1, w, (int64_t)1, (uint32_t)n);
+  x.run(k_first_nonempty, groups(n, kT), rp, n, w);
+  const uint32_t m = x.read_u32(w);
+  if (m >= (uint32_t)n) return kOpsEmptyRow;  // every row empty
+  x.run(k_fill_u32, groups(n, kT), anc, (int64_t)n, ~0u);
+  for (int64_t pass = 0;; ++pass) {  // a value falls along one more entry each pass at least
+    if (pass > (int64_t)n) return kOpsDevice;
+    x.run(k_fill_u32, 1, w + 1, (int64_t)1, 0u);
+    x.run(k_anc_pass, groups(n, kT), rp, ci, n, anc, w + 1);
+    if (!x.read_u32(w + 1)) break;
+    if (x.failed()) return kOpsDevice;
+  }
+  x.run(k_fill_u32, 1, w + 2, (int64_t)1, 0u);
+  x.run(k_dir_roots, groups(n, kT), (const uint32_t*)anc, n, m, root, w + 2);
+  if (x.read_u32(w + 2)) return kOpsEmptyRow;
+  return x.failed() ? kOpsDevice : kOpsOk;
+}
+
+// convert + prepareRCM(check_sym) (src/CSR.cpp:170-208, include/Graph.h:74-264). A structurally symmetric result
+// is repaired by a union-find over its components (an empty row is the reference's non-terminating case); any
+// other by the directed characterisation above. *links_out: links added.
 template <class X, class V>
 int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r, const int32_t* c, const V* vals,
                  bool check_sym, Csr<V>& out, int64_t* links_out) {
@@ -759,32 +945,17 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
   int rc = coo_to_csr(x, rows, cols, nnz, r, c, vals, a, &skeys);
   if (rc) return rc;
   bool symmetric_known = false;  // the pair pass decided structural symmetry
+  bool nonsym = false;           // the result is not structurally symmetric: the directed repair runs
   if (check_sym && nnz > 0) {
-    uint32_t* pos = x.template alloc<uint32_t>((size_t)nnz + 1);
     uint32_t* tot = x.template alloc<uint32_t>(2);
-    if (x.failed()) return kOpsDevice;
-    x.run(k_offdiag_flags, groups(nnz, kT), (const uint64_t*)skeys, nnz, pos);
-    x.run(k_fill_u32, 1, pos + nnz, (int64_t)1, 0u);
-    scan_u32(x, pos, pos, nnz + 1, tot);
-    const uint32_t m = x.read_u32(tot);
     uint32_t* odd = x.template alloc<uint32_t>((size_t)nnz + 1);
     if (x.failed()) return kOpsDevice;
-    x.run(k_fill_u32, groups(nnz + 1, kT), odd, nnz + 1, 0u);
     x.run(k_fill_u32, 1, tot, (int64_t)2, 0u);
-    if (m > 0) {
-      uint64_t* pk = x.template alloc<uint64_t>(m);
-      uint64_t* pk2 = x.template alloc<uint64_t>(m);
-      uint32_t* pidx = x.template alloc<uint32_t>(m);
-      uint32_t* pidx2 = x.template alloc<uint32_t>(m);
-      if (x.failed()) return kOpsDevice;
-      x.run(k_pair_keys, groups(nnz, kT), (const uint64_t*)skeys, nnz, (const uint32_t*)pos, pk, pidx);
-      const int nb = bits_of((uint64_t)(rows > cols ? rows : cols));
-      radix_sort(x, pk, pidx, pk2, pidx2, (int64_t)m, nb, nb);
-      x.run(k_pair_odd, groups(m, kT), (const uint64_t*)pk, (const uint32_t*)pidx, (const uint64_t*)skeys,
-            (int64_t)m, odd, tot);
-    }
-    if (x.read_u32(tot)) return kOpsNotSymmetric;
-    symmetric_known = true;
+    x.run(k_fill_u32, 1, odd + nnz, (int64_t)1, 0u);
+    x.run(k_pair_parity, groups(nnz, kT), (const uint64_t*)skeys, (const int32_t*)a.rp, (const int32_t*)a.ci, rows,
+          nnz, odd, tot);
+    nonsym = x.read_u32(tot) != 0;
+    symmetric_known = !nonsym;
     scan_u32(x, odd, odd, nnz + 1, tot + 1);
     const uint32_t extra = x.read_u32(tot + 1);
     if (x.failed()) return kOpsDevice;
@@ -813,14 +984,18 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
   x.run(k_fill_u32, 1, flag, (int64_t)2, 0u);
   if (symmetric_known) {
     x.run(k_empty_rows, groups(n, kT), (const int32_t*)a.rp, n, flag);
-  } else {
+  } else if (!nonsym) {
     x.run(k_struct_checks, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, flag);
-    if (x.read_u32(flag + 1)) return kOpsNotSymmetric;
+    nonsym = x.read_u32(flag + 1) != 0;
   }
-  if (x.read_u32(flag)) return kOpsEmptyRow;
-  x.run(k_cc_init, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, root);
-  x.run(k_cc_hook, groups(n, kT), (const int32_t*)a.rp, (const int32_t*)a.ci, n, root);
-  x.run(k_cc_finish, groups(n, kT), root, n);
+  if (nonsym) {
+    rc = directed_roots(x, n, (const int32_t*)a.rp, (const int32_t*)a.ci, root);
+    if (rc) return rc;
+  } else {
+    if (x.read_u32(flag)) return kOpsEmptyRow;
+    rc = components(x, n, (const int32_t*)a.rp, (const int32_t*)a.ci, root);
+    if (rc) return rc;
+  }
   uint32_t* tot = x.template alloc<uint32_t>(1);
   int32_t* rp2 = x.template keep<int32_t>((size_t)n + 1);
   if (x.failed()) return kOpsDevice;

@@ -12,7 +12,8 @@ import oracle_lib as O
 import parallelbatchrcm_amd as P
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-ORDERS = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if not os.path.basename(p).startswith("prepare"))
+ORDERS = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                if not os.path.basename(p).startswith(("prepare", "ingest")))
 
 
 def _load(path):
@@ -70,3 +71,22 @@ def test_batch_reproduces_reference_fixtures():
         perms = b.perms()
     for k, g in enumerate(gs):
         assert starts[k] == int(g["start"]) and np.array_equal(perms[k], g["perm"]) and bw[k] == int(g["bandwidth"])
+
+
+@pytest.mark.parametrize("name", ["ingest_sym_grid", "ingest_pattern_sym", "ingest_general_nonsym"])
+def test_ingest_fixtures_match_oracle_and_host_load(tmp_path, name):
+    """The file-ingest fixtures (the reference's Graph<double>::load of each file) equal the oracle's
+    prepareRCM of the parsed file and the library's host Graph::load."""
+    g = _load(os.path.join(GOLDEN, f"{name}.npz"))
+    path = tmp_path / "g.mtx"
+    path.write_bytes(g["text"].tobytes())
+    m, _ = P.loadMTX(path, np.float64)
+    data = np.ones(m.nnz) if m.data is None else m.data
+    added, rp, ci, vals = O.prepare_values(m.row_offsets, m.col_ids, data, True)
+    assert added >= 0
+    np.testing.assert_array_equal(rp, g["row_ptr"])
+    np.testing.assert_array_equal(ci, g["col_idx"])
+    np.testing.assert_array_equal(vals, g["values"])
+    loaded = P.Graph.load(path, np.float64, write_cache=False)
+    np.testing.assert_array_equal(loaded.csr.col_ids, g["col_idx"])
+    np.testing.assert_array_equal(np.asarray(loaded.csr.data, np.float64), g["values"])

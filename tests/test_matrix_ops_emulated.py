@@ -1,7 +1,7 @@
 """CPU checks of the canonicalisation and permutation kernels (parallelbatchrcm_amd/csrc/matrix_ops.hpp) run in
 the wave emulator through the library's own orchestration: every lane is a host thread, ballots and barriers are
-collectives. Checked against numpy definitions and the host C++ convert / prepareRCM, which tests/test_formats.py
-pins to the reference's own code."""
+collectives. Checked against numpy definitions and the oracle's convert / value-carrying prepareRCM
+(oracle/rcm_oracle.c, pinned to the reference's own Graph<double>::prepareRCM in tests/test_oracle.py)."""
 from __future__ import annotations
 
 import numpy as np
@@ -88,25 +88,71 @@ def test_canonicalize_matches_host(seed, check_sym):
     rc, res = E.canonicalize(n, n, r, c, v, check_sym)
     assert rc == 0
     nn, rp, ci, vv, links = res
-    host, hlinks = P.prepare_rcm(P.convert_coo(n, n, r, c, v), check_sym)
-    assert links == hlinks
-    np.testing.assert_array_equal(rp, host.row_offsets)
-    np.testing.assert_array_equal(ci, host.col_ids)
-    np.testing.assert_array_equal(vv, host.data)
+    _expect_oracle(n, r, c, v, check_sym, rp, ci, vv, links)
 
 
-def test_canonicalize_refusals():
-    # a pair stored twice in one direction is "symmetric" to the reference's parity check but not structurally
+def _expect_oracle(n, r, c, v, check_sym, rp, ci, vv, links):
+    orp, oci, ov = O.convert(n, r, c, v)
+    added, erp, eci, ev = O.prepare_values(orp, oci, ov, check_sym)
+    assert links == added
+    np.testing.assert_array_equal(rp, erp)
+    np.testing.assert_array_equal(ci, eci)
+    np.testing.assert_array_equal(vv, ev)
+
+
+def directed_coo(rng, n, k, diag=True, cycles=0):
+    """Random directed entries (no mirror), optionally a full diagonal, and `cycles` directed cycles."""
+    r = list(rng.integers(0, n, k))
+    c = list(rng.integers(0, n, k))
+    if diag:
+        r += list(range(n))
+        c += list(range(n))
+    for _ in range(cycles):
+        nodes = rng.choice(n, int(rng.integers(2, 6)), replace=False)
+        for a, b in zip(nodes, np.roll(nodes, -1)):
+            r.append(a)
+            c.append(b)
+    return np.array(r, np.int32), np.array(c, np.int32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("check_sym", [True, False])
+def test_canonicalize_directed_matches_oracle(seed, check_sym):
+    """Matrices that stay structurally non-symmetric (one-sided entries without the symmetrisation, or pairs
+    stored an even number of times in one direction with it): the repair follows the stored rows as the
+    reference's walk does (include/Graph.h:160-263)."""
+    rng = np.random.default_rng(70 + seed)
+    n = int(rng.integers(40, 400))
+    r, c = directed_coo(rng, n, n // 2 + int(rng.integers(0, n)), diag=True, cycles=seed)
+    if check_sym:  # doubled one-sided entries keep the result non-symmetric after the parity pass
+        r = np.concatenate([r, r[:7]])
+        c = np.concatenate([c, c[:7]])
+    v = rng.integers(-99, 99, r.size).astype(np.float64)
+    rc, res = E.canonicalize(n, n, r, c, v, check_sym)
+    assert rc == 0, rc
+    nn, rp, ci, vv, links = res
+    _expect_oracle(n, r, c, v, check_sym, rp, ci, vv, links)
+
+
+def test_canonicalize_closed_forms():
+    # a pair stored twice in one direction is "symmetric" to the reference's parity check but not structurally;
+    # 0 -> 1 -> 2 reaches everything: nothing to link
     r = np.array([0, 0, 0, 1, 2, 1], np.int32)
     c = np.array([1, 1, 0, 1, 2, 2], np.int32)
-    rc, _ = E.canonicalize(3, 3, r, c, None, True)
-    assert rc == -12
-    # without the symmetrisation the structure check finds it
-    rc, _ = E.canonicalize(3, 3, np.array([0, 0, 1, 2], np.int32), np.array([0, 1, 1, 2], np.int32), None, False)
-    assert rc == -12
-    # the host repair follows the directed rows there and still succeeds
-    _, links = P.prepare_rcm(P.convert_coo(3, 3, r, c, None), True)
-    assert links >= 0
+    rc, res = E.canonicalize(3, 3, r, c, None, True)
+    assert rc == 0 and res[4] == 0
+    # without the symmetrisation: 0 -> 1, 2 alone with its diagonal: one link (1, 2)
+    rc, res = E.canonicalize(3, 3, np.array([0, 0, 1, 2], np.int32), np.array([0, 1, 1, 2], np.int32), None, False)
+    assert rc == 0 and res[4] == 1
+    assert res[2].tolist() == [0, 1, 1, 2, 1, 2] and res[1].tolist() == [0, 2, 4, 6]
+    # a directed chain 2 -> 1 -> 0 with diagonals: 0 reaches nothing, so 1 and 2 are each linked
+    rc, res = E.canonicalize(3, 3, np.array([0, 1, 1, 2, 2], np.int32), np.array([0, 0, 1, 1, 2], np.int32), None,
+                             False)
+    assert rc == 0 and res[4] == 2
+    # a node no smaller node reaches and on no cycle (no diagonal): the reference's walk never ends
+    rc, _ = E.canonicalize(3, 3, np.array([0, 2], np.int32), np.array([0, 0], np.int32), None, False)
+    assert rc == -13
+    assert O.prepare(np.array([0, 1, 1, 2], np.int32), np.array([0, 0], np.int32), False)[0] == -3
     # an empty row: the reference's repair loop does not terminate
     rc, _ = E.canonicalize(3, 3, np.array([0, 0, 1], np.int32), np.array([0, 1, 0], np.int32), None, True)
     assert rc == -13
