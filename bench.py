@@ -38,32 +38,52 @@ def alg_bytes(n: int, nnz: int) -> int:
     return 4 * nnz + 12 * n + 4
 
 
-def cpu_baseline(timeout_s: int = 240):
+CPU_SHARE = 16  # host cores one GPU's jobs may use on the GPU host
+
+
+def cpu_baseline(timeout_s: int = 400):
     """Reference CPU paths timed on this host, in a child process (the reference's CPU_BATCH is third-party
-    code; a crash there must not end the benchmark)."""
+    code; a crash there must not end the benchmark): on the 128^3 stencil the serial rcm_seq<true> and CPU_BATCH
+    (batch size 512) at every thread count t = 1 .. the host share, 25 iterations each (src/main.cpp:456-464,
+    src/rcm_cpu.cpp:113-191); on the 100 x 100 2D stencil (the reference's CPU-path configuration) the serial
+    path and CPU_BATCH at the best t."""
     code = r"""
 import json, os, sys, time
 sys.path.insert(0, %r); sys.path.insert(0, os.path.join(%r, 'tests'))
 import numpy as np, oracle_lib as O, parallelbatchrcm_amd as P
 m = P.workloads.config(2, seed=%d)
 rc, s, it, _ = O.peripheral(m.row_offsets, m.col_ids)
-threads = max(1, min(16, os.cpu_count() or 1))
-out = {"n": m.rows, "nnz": m.nnz, "threads": threads}
+try:
+    affinity = len(os.sched_getaffinity(0))
+except Exception:
+    affinity = os.cpu_count() or 1
+share = max(1, min(%d, affinity))
+out = {"n": m.rows, "nnz": m.nnz, "nproc": os.cpu_count(), "affinity": affinity, "share": share}
 if O.ref_available():
     _, ser = O.ref_rcm(m.row_offsets, m.col_ids, s, threads=1, batchsize=1)
-    _, bat = O.ref_rcm(m.row_offsets, m.col_ids, s, threads=threads, batchsize=512)
-    out.update(kind="reference", serial_ms=ser, batch_ms=bat)
+    sweep = {}
+    for t in range(1, share + 1):
+        _, sweep[t] = O.ref_rcm(m.row_offsets, m.col_ids, s, threads=t, batchsize=512)
+        print("CPUSWEEP", t, sweep[t], flush=True)
+    best = min(sweep, key=sweep.get)
+    c1 = P.workloads.config(1, seed=1)
+    rc1, s1, _, _ = O.peripheral(c1.row_offsets, c1.col_ids)
+    _, c1_ser = O.ref_rcm(c1.row_offsets, c1.col_ids, s1, threads=1, batchsize=1)
+    _, c1_bat = O.ref_rcm(c1.row_offsets, c1.col_ids, s1, threads=best, batchsize=512)
+    out.update(kind="reference", serial_ms=ser, batch_ms=sweep[best], best_threads=best,
+               batch_sweep_ms={str(t): v for t, v in sweep.items()},
+               c1={"n": c1.rows, "nnz": c1.nnz, "serial_ms": c1_ser, "batch_ms": c1_bat, "threads": best})
 else:
     ts = []
     for _ in range(5):
         t = time.perf_counter(); O.cm(m.row_offsets, m.col_ids, s); ts.append((time.perf_counter() - t) * 1e3)
-    out.update(kind="port", serial_ms=sum(ts) / len(ts), batch_ms=None)
+    out.update(kind="port", serial_ms=sum(ts) / len(ts), batch_ms=None, best_threads=1)
 try:
     out["cpu_model"] = [l.split(':', 1)[1].strip() for l in open('/proc/cpuinfo') if l.startswith('model name')][0]
 except Exception:
     out["cpu_model"] = "unknown"
 print("CPUBASE " + json.dumps(out))
-""" % (ROOT, ROOT, SEED_C2)
+""" % (ROOT, ROOT, SEED_C2, CPU_SHARE)
     try:
         res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=timeout_s)
     except subprocess.TimeoutExpired:
@@ -120,12 +140,10 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, a
         tm = b.timing()
         out.update(run_ms_mean=float(np.mean(devs)), run_wall_ms_mean=float(np.mean(walls)) * 1e3,
                    matrices_per_s=count / float(np.mean(walls)), nnz_per_s=out["sum_nnz"] / float(np.mean(walls)),
-                   batches=int(tm.batches), graph_replays=int(tm.graph_replays))
-        # sanity: every order is a permutation of its matrix
-        for k in range(0, count, max(1, count // 64)):
-            pk = perms[k]
-            if pk.shape[0] != bs.n[k] or np.bincount(pk, minlength=pk.shape[0]).max() != 1:
-                raise RuntimeError(f"matrix {k}: not a permutation")
+                   batches=int(tm.batches), graph_replays=int(tm.graph_replays),
+                   end_to_end_matrices_per_s=count / e2e_s)
+    # every order of the first run checked on the GPU (outside the timed regions)
+    out["verified"] = verify_all([bs.matrix(k) for k in range(count)], starts, perms)
     # one-at-a-time plain calls on a seeded sample (inputs resident, start known: the same work per matrix)
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(count, size=min(sample, count), replace=False))
@@ -145,10 +163,14 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, a
     return out
 
 
-def bench_serving(requests: int, rates, seed: int, lanes: int):
-    """Serving form: a seeded Poisson stream of reorder requests (95% the batched mix, 5% R-MAT scale 18-20,
-    a,b,c = .57,.19,.19, symmetrised, diagonal, connected) through rcm_server; latency is arrival -> result
-    on the host. One run per arrival rate."""
+def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int):
+    """Serving form: a seeded stream of reorder requests (95% the batched mix, 5% R-MAT scale 18-20,
+    a,b,c = .57,.19,.19, symmetrised, diagonal, connected) through rcm_server. The same requests are first served
+    one at a time on equal terms: each alone, with upload, GPU start search, reorder and download (a warm
+    context; the plain one-shot call rcm_gpu_batch, which builds its context per call as GPU_BATCH::rcm does,
+    on a sample). Capacity is the server's throughput with every request submitted at once; the latency runs
+    use Poisson arrivals at fractions of that capacity (latency: arrival -> result on the host). Every result of
+    the capacity run is checked on the GPU afterwards."""
     import threading
 
     import numpy as np
@@ -163,13 +185,33 @@ def bench_serving(requests: int, rates, seed: int, lanes: int):
     reqs = pool + [rmats[i % 3] for i in range(requests - nmix)]
     order = rng.permutation(len(reqs))
     reqs = [reqs[i] for i in order]
-    out = {"requests": len(reqs), "rmat_requests": requests - nmix, "lanes": lanes, "runs": []}
-    for rate in rates:
-        gaps = rng.exponential(1.0 / rate, size=len(reqs))
-        lat, dev, que = [], [], []
+    out = {"requests": len(reqs), "rmat_requests": requests - nmix, "lanes": lanes,
+           "sum_nnz": int(sum(m.nnz for m in reqs)), "runs": []}
+
+    # one at a time, each request alone on the GPU (warm context)
+    with P.RcmContext(timeout_ms=60000) as c:
+        c.load(reqs[0])
+        c.run(c.find_start())
+        t0 = time.perf_counter()
+        for m in reqs:
+            c.load(m)
+            c.run(c.find_start())
+            c.perm()
+        warm = time.perf_counter() - t0
+    out["one_at_a_time_per_s"] = len(reqs) / warm
+    sample = reqs[: min(cold_sample, len(reqs))]
+    t0 = time.perf_counter()
+    for m in sample:
+        P.rcm_gpu_batch(m, -1, P.RCM_FLAG_STABLE)
+    out["one_shot_call_per_s"] = len(sample) / (time.perf_counter() - t0)
+    out["one_shot_call_sample"] = len(sample)
+    print(f"[bench] serving baseline: one at a time {out['one_at_a_time_per_s']:.1f} requests/s (warm context), "
+          f"one-shot calls {out['one_shot_call_per_s']:.1f}/s", file=sys.stderr, flush=True)
+
+    def serve(gaps):
+        lat, dev, que, res = [], [], [], [None] * len(reqs)
         with P.RcmServer(lanes=lanes) as srv:
             ids = [None] * len(reqs)
-            done = threading.Event()
 
             def collect():
                 for i in range(len(reqs)):
@@ -181,28 +223,45 @@ def bench_serving(requests: int, rates, seed: int, lanes: int):
                     lat.append(js.latency_ms)
                     dev.append(js.device_ms)
                     que.append(js.queue_ms)
-                done.set()
+                    res[i] = (js.start_node, perm)
 
             th = threading.Thread(target=collect)
             th.start()
             t0 = time.perf_counter()
             due = t0
             for i, m in enumerate(reqs):
-                due += gaps[i]
-                dt = due - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
+                if gaps is not None:
+                    due += gaps[i]
+                    dt = due - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
                 ids[i] = srv.submit(m, -1, P.RCM_FLAG_STABLE)
             th.join()
             wall = time.perf_counter() - t0
             st = srv.stats()
+        return wall, lat, dev, que, st, res
+
+    wall, lat, dev, que, st, res = serve(None)
+    cap = len(reqs) / wall
+    out["capacity_per_s"] = cap
+    out["capacity_p50_ms"] = float(np.percentile(lat, 50))
+    out["capacity_p99_ms"] = float(np.percentile(lat, 99))
+    out["capacity_groups"] = int(st.loop_iterations)
+    out["speedup_vs_one_at_a_time"] = cap / out["one_at_a_time_per_s"]
+    print(f"[bench] serving capacity {cap:.1f} requests/s ({out['speedup_vs_one_at_a_time']:.1f}x one at a time)",
+          file=sys.stderr, flush=True)
+    out["verified"] = verify_all(reqs, [r[0] for r in res], [r[1] for r in res])
+    for f in fracs:
+        rate = f * cap
+        gaps = rng.exponential(1.0 / rate, size=len(reqs))
+        wall, lat, dev, que, st, _ = serve(gaps)
         out["runs"].append({
-            "arrival_rate_per_s": rate, "throughput_per_s": len(reqs) / wall,
+            "fraction_of_capacity": f, "arrival_rate_per_s": rate, "throughput_per_s": len(reqs) / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(np.mean(lat)), "p99_queue_ms": float(np.percentile(que, 99)),
             "p50_device_ms": float(np.percentile(dev, 50)), "max_inflight": int(st.max_inflight),
             "groups": int(st.loop_iterations)})
-        print(f"[bench] serving rate {rate}/s: p50 {out['runs'][-1]['p50_ms']:.1f} ms "
+        print(f"[bench] serving at {f:.2f} of capacity ({rate:.0f}/s): p50 {out['runs'][-1]['p50_ms']:.1f} ms "
               f"p99 {out['runs'][-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     return out
 
@@ -291,17 +350,65 @@ def bench_variants_rmat(scale: int, steps: int, seed: int):
             for _ in range(steps + 1):
                 ctx.run(s)
                 ts.append(ctx.timing().run_kernel_ms)
-            perm = ctx.perm()
-            ref = perm if ref is None else ref
             out[f"variant{v}_ms"] = float(np.median(ts[1:]))
-            out[f"variant{v}_same_order"] = bool(np.array_equal(perm, ref))
             out[f"variant{v}_steps"] = ctx.timing().batches  # batches (0-4) or BFS levels (5)
+            # the order checked on the device against the characterisation of the stable CM order (after timing)
+            out[f"variant{v}_verified"] = ctx.verify(s) == 0
         ctx.set_param("variant", 0)
         tune = ctx.autotune(s, repeats=2)
         out["autotune_variant"] = tune.variant
         out["autotune_ms"] = tune.best_ms
     print(f"[bench] R-MAT {scale}: variant 0 {out['variant0_ms']:.2f} ms, 4 {out['variant4_ms']:.2f} ms, "
           f"level-synchronous {out['variant5_ms']:.2f} ms; autotune picks {tune.variant}", file=sys.stderr, flush=True)
+    if not all(out[f"variant{v}_verified"] for v in (0, 4, 5)):
+        raise RuntimeError(f"R-MAT {scale}: an engine variant's order failed the device check")
+    return out
+
+
+def verify_all(mats, starts, perms, flags=0):
+    """Checks every order on the GPU (rcm_ctx_verify: the characterisation of the stable CM order, O(n + nnz)),
+    outside any timed region. Returns the number of orders checked; raises on the first that fails."""
+    import parallelbatchrcm_amd as P
+
+    with P.RcmContext() as c:
+        for k, (m, s, p) in enumerate(zip(mats, starts, perms)):
+            c.load(m)
+            r = c.verify(int(s), p, flags)
+            if r != 0:
+                raise RuntimeError(f"order {k} (n={m.rows}) is not the stable CM order from {s}: position {r - 1}")
+    return len(mats)
+
+
+def bench_single(which: int, scale: int, seed: int, steps: int, name: str):
+    """A single-matrix leg: kernel milliseconds (autotuned engine), end-to-end time and the device check."""
+    import numpy as np
+
+    import parallelbatchrcm_amd as P
+
+    m = P.workloads.config(which, scale=scale, seed=seed)
+    with P.RcmContext(timeout_ms=30000) as ctx:
+        w0 = time.perf_counter()
+        ctx.load(m)
+        s = ctx.find_start()
+        ctx.run(s, P.RCM_FLAG_REVERSE)
+        bw = ctx.bandwidth()
+        ctx.perm()
+        e2e = (time.perf_counter() - w0) * 1e3
+        tm = ctx.timing()
+        tune = ctx.autotune(s, repeats=3)
+        ts = []
+        for _ in range(steps):
+            ctx.run(s)
+            ts.append(ctx.timing().run_kernel_ms)
+        ok = ctx.verify(s) == 0
+    k = float(np.mean(ts))
+    out = {"workload": name, "n": m.rows, "nnz": m.nnz, "start_node": s, "kernel_ms": k,
+           "kernel_nnz_per_s": m.nnz / (k * 1e-3), "achieved_GBs": alg_bytes(m.rows, m.nnz) / (k * 1e-3) / 1e9,
+           "end_to_end_warm_ms": e2e, "start_search_ms": tm.start_search_ms, "start_sweeps": tm.start_iterations,
+           "bandwidth": bw, "variant": tune.variant, "workers": tune.workers, "verified": ok}
+    print(f"[bench] {name}: kernel {k:.3f} ms, end to end {e2e:.1f} ms, verified {ok}", file=sys.stderr, flush=True)
+    if not ok:
+        raise RuntimeError(f"{name}: the order failed the device check")
     return out
 
 
@@ -369,9 +476,12 @@ def main() -> int:
                     help="batched form on COUNT matrices of the mix (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--one-at-a-time-sample", type=int, default=256)
-    ap.add_argument("--serving", type=int, default=400, help="serving form: requests per arrival rate (0 = skip)")
-    ap.add_argument("--serving-rates", type=str, default="200,1000,4000")
+    ap.add_argument("--serving", type=int, default=400, help="serving form: requests per run (0 = skip)")
+    ap.add_argument("--serving-fracs", type=str, default="0.25,0.5,0.75",
+                    help="Poisson arrival rates as fractions of the measured capacity")
     ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--serving-cold-sample", type=int, default=48)
+    ap.add_argument("--c3", type=int, default=1, help="the 1M-point random geometric graph leg (0 = skip)")
     ap.add_argument("--rmat", type=int, default=20, help="R-MAT scale of the engine-variant leg (0 = skip)")
     ap.add_argument("--pipeline", type=int, default=5,
                     help="steps of the canonicalise and permute legs on the 128^3 stencil (0 = skip)")
@@ -476,6 +586,10 @@ def main() -> int:
     if args.pipeline > 0:
         line["pipeline"] = bench_pipeline(m, perm, args.pipeline, 9)
 
+    if args.c3:
+        line["c3"] = bench_single(3, 0, 3, args.steps, "random geometric graph, 1M points in the unit square, mean "
+                                  "degree ~12, relabelled and repaired")
+
     if args.rmat > 0:
         line["variants_rmat"] = bench_variants_rmat(args.rmat, 3, args.rmat)
 
@@ -485,24 +599,31 @@ def main() -> int:
                                         args.autotune)
 
     if args.serving > 0:
-        rates = [float(x) for x in args.serving_rates.split(",") if x]
-        print(f"[bench] serving form: {args.serving} requests at {rates} /s", file=sys.stderr, flush=True)
-        line["serving"] = bench_serving(args.serving, rates, 5, args.serving_lanes)
+        fracs = [float(x) for x in args.serving_fracs.split(",") if x]
+        print(f"[bench] serving form: {args.serving} requests", file=sys.stderr, flush=True)
+        line["serving"] = bench_serving(args.serving, fracs, 5, args.serving_lanes, args.serving_cold_sample)
 
     if not args.no_cpu:
+        print("[bench] CPU baseline (reference CPU paths on this host)", file=sys.stderr, flush=True)
         cb = cpu_baseline()
         if cb:
             best_ms = cb["batch_ms"] if cb.get("batch_ms") else cb["serial_ms"]
-            cores = cb["threads"] if cb.get("batch_ms") else 1
+            cores = cb["best_threads"] if cb.get("batch_ms") else 1
             line["cpu_baseline"] = {
                 "value": cb["nnz"] / (best_ms * 1e-3),
                 "unit": "nnz/s",
                 "cores": cores,
                 "kind": cb["kind"],
-                "sample": (f"same 128^3 matrix and start; reference RealCPU::rcm, 25 iterations mean: "
-                           f"CPU_BATCH {cores} threads batch 512 = {cb.get('batch_ms')} ms, "
-                           f"serial = {cb['serial_ms']} ms; host {cb.get('cpu_model')}"),
+                "sample": (f"same 128^3 matrix and start; reference RealCPU::rcm, 25 iterations mean each: "
+                           f"CPU_BATCH batch 512 at t = 1..{cb['share']} threads (host nproc {cb['nproc']}, "
+                           f"affinity {cb['affinity']}, share {cb['share']}), best t = {cores}: {best_ms:.2f} ms; "
+                           f"serial = {cb['serial_ms']:.2f} ms; host {cb.get('cpu_model')}"),
                 "serial_nnz_per_s": cb["nnz"] / (cb["serial_ms"] * 1e-3),
+                "nproc": cb["nproc"],
+                "batch_sweep_ms": cb.get("batch_sweep_ms"),
+                "c1_cpu": cb.get("c1"),
+                "gpu_over_cpu_batch": (cb["nnz"] / (best_ms * 1e-3)) and line["kernel_nnz_per_s"] / (
+                    cb["nnz"] / (best_ms * 1e-3)),
             }
         else:
             line["cpu_baseline"] = None
