@@ -111,18 +111,22 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, a
     e2e0 = time.perf_counter()
     with P.RcmBatch(timeout_ms=120000, graph=True) as b:
         b.load(bs)
-        starts, iters, st = b.find_start()
+        starts, iters, st = b.order()  # start search and order in one call
         if not (st == 0).all():
-            raise RuntimeError(f"batched start search failed for {int((st != 0).sum())} matrices")
-        st = b.run(starts)
+            raise RuntimeError(f"batched search and order failed for {int((st != 0).sum())} matrices")
         bw = b.bandwidth()
         perms = b.perms()
         e2e_s = time.perf_counter() - e2e0
-        if not (st == 0).all():
-            raise RuntimeError("batched run failed")
         tm = b.timing()
-        out.update(load_ms=tm.load_ms, start_search_ms=tm.start_search_ms, start_sweeps=tm.start_sweeps,
-                   end_to_end_s=e2e_s, workers=tm.workers)
+        out.update(load_ms=tm.load_ms, order_ms=tm.order_ms, order_launches=int(tm.order_passes),
+                   end_to_end_s=e2e_s, workers=tm.workers,
+                   sweeps_histogram={str(int(v)): int(c) for v, c in zip(*np.unique(iters, return_counts=True))})
+        # the two-call path's search alone (find_start; the run is timed below), for comparison
+        s2, _, st2 = b.find_start()
+        if not (np.array_equal(s2, starts) and (st2 == 0).all()):
+            raise RuntimeError("batched start search disagrees with the fused search")
+        tm = b.timing()
+        out.update(start_search_ms=tm.start_search_ms, start_sweeps=tm.start_sweeps)
         if autotune:
             t0 = time.perf_counter()
             r = b.autotune(starts)
@@ -163,7 +167,7 @@ def bench_batched(count: int, seed: int, steps: int, warmup: int, sample: int, a
     return out
 
 
-def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int):
+def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int, workers: int = 0):
     """Serving form: a seeded stream of reorder requests (95% the batched mix, 5% R-MAT scale 18-20,
     a,b,c = .57,.19,.19, symmetrised, diagonal, connected) through rcm_server. The same requests are first served
     one at a time on equal terms: each alone, with upload, GPU start search, reorder and download (a warm
@@ -191,11 +195,11 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int)
     # one at a time, each request alone on the GPU (warm context)
     with P.RcmContext(timeout_ms=60000) as c:
         c.load(reqs[0])
-        c.run(c.find_start())
+        c.order()
         t0 = time.perf_counter()
         for m in reqs:
             c.load(m)
-            c.run(c.find_start())
+            c.order()  # start search and order in one call, as the server does per group
             c.perm()
         warm = time.perf_counter() - t0
     out["one_at_a_time_per_s"] = len(reqs) / warm
@@ -210,7 +214,7 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int)
 
     def serve(gaps):
         lat, dev, que, res = [], [], [], [None] * len(reqs)
-        with P.RcmServer(lanes=lanes) as srv:
+        with P.RcmServer(lanes=lanes, workers=workers) as srv:
             ids = [None] * len(reqs)
 
             def collect():
@@ -247,6 +251,11 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int)
     out["capacity_p50_ms"] = float(np.percentile(lat, 50))
     out["capacity_p99_ms"] = float(np.percentile(lat, 99))
     out["capacity_groups"] = int(st.loop_iterations)
+    g = max(1, int(st.loop_iterations))
+    out["capacity_group_breakdown_ms"] = {  # per group, mean: where a group's host wall time goes
+        "host_copy": st.group_host_copy_ms / g, "load_h2d_validate": st.group_load_ms / g,
+        "search_and_order": st.group_order_ms / g, "bandwidth_d2h": st.group_readback_ms / g,
+        "deliver": st.group_deliver_ms / g, "launches": st.group_launches / g}
     out["speedup_vs_one_at_a_time"] = cap / out["one_at_a_time_per_s"]
     print(f"[bench] serving capacity {cap:.1f} requests/s ({out['speedup_vs_one_at_a_time']:.1f}x one at a time)",
           file=sys.stderr, flush=True)
@@ -260,7 +269,8 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int)
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(np.mean(lat)), "p99_queue_ms": float(np.percentile(que, 99)),
             "p50_device_ms": float(np.percentile(dev, 50)), "max_inflight": int(st.max_inflight),
-            "groups": int(st.loop_iterations)})
+            "groups": int(st.loop_iterations),
+            "group_order_ms_mean": st.group_order_ms / max(1, int(st.loop_iterations))})
         print(f"[bench] serving at {f:.2f} of capacity ({rate:.0f}/s): p50 {out['runs'][-1]['p50_ms']:.1f} ms "
               f"p99 {out['runs'][-1]['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     return out
@@ -389,12 +399,13 @@ def bench_single(which: int, scale: int, seed: int, steps: int, name: str):
     with P.RcmContext(timeout_ms=30000) as ctx:
         w0 = time.perf_counter()
         ctx.load(m)
-        s = ctx.find_start()
-        ctx.run(s, P.RCM_FLAG_REVERSE)
+        s, _ = ctx.order(flags=P.RCM_FLAG_STABLE | P.RCM_FLAG_REVERSE)
         bw = ctx.bandwidth()
         ctx.perm()
         e2e = (time.perf_counter() - w0) * 1e3
         tm = ctx.timing()
+        ctx.find_start()  # the two-call path's search alone, for comparison
+        tm_search = ctx.timing().start_search_ms
         tune = ctx.autotune(s, repeats=3)
         ts = []
         for _ in range(steps):
@@ -404,8 +415,9 @@ def bench_single(which: int, scale: int, seed: int, steps: int, name: str):
     k = float(np.mean(ts))
     out = {"workload": name, "n": m.rows, "nnz": m.nnz, "start_node": s, "kernel_ms": k,
            "kernel_nnz_per_s": m.nnz / (k * 1e-3), "achieved_GBs": alg_bytes(m.rows, m.nnz) / (k * 1e-3) / 1e9,
-           "end_to_end_warm_ms": e2e, "start_search_ms": tm.start_search_ms, "start_sweeps": tm.start_iterations,
-           "bandwidth": bw, "variant": tune.variant, "workers": tune.workers, "verified": ok}
+           "end_to_end_warm_ms": e2e, "order_ms": tm.order_ms, "order_passes": tm.order_passes,
+           "start_search_ms": tm_search, "start_sweeps": tm.start_iterations, "bandwidth": bw,
+           "variant": tune.variant, "workers": tune.workers, "verified": ok}
     print(f"[bench] {name}: kernel {k:.3f} ms, end to end {e2e:.1f} ms, verified {ok}", file=sys.stderr, flush=True)
     if not ok:
         raise RuntimeError(f"{name}: the order failed the device check")
@@ -415,28 +427,58 @@ def bench_single(which: int, scale: int, seed: int, steps: int, name: str):
 COPY_BYTES = 256 << 20  # calibration copy for the HBM traffic counters
 
 
-def pmc_probe(runs: int, claim_filter: int = -1, variant: int = -1, workers: int = 0) -> int:
+def pmc_probe(runs: int, claim_filter: int = -1, variant: int = -1, workers: int = 0, legs: str = "single") -> int:
     """Workload of the rocprofv3 --pmc passes: the 128^3 stencil reorder `runs` times, then `runs` elementwise
     kernels that read and write COPY_BYTES each (a plain copy of known traffic) to calibrate the counters. The
     engine configuration is given explicitly (variant, workgroups), set to what the timed leg's autotuner picks:
-    tuning under the counters would time perturbed runs."""
+    tuning under the counters would time perturbed runs. `legs` may add "batched" (the 4096-matrix mix, `runs`
+    launches of k_reorder_many with the configuration the batched leg's autotuner picks: variant 4, 256
+    workgroups, one helper per 131072 nodes) and "pipeline" (`runs` canonicalisations of the 128^3 stencil's
+    lower triangle, the pipeline leg's input)."""
+    import numpy as np
     import torch
 
     import parallelbatchrcm_amd as P
 
+    legs = set(legs.split(","))
     m = P.workloads.config(2, seed=SEED_C2)
-    with P.RcmContext(timeout_ms=30000) as ctx:
-        if claim_filter >= 0:
-            ctx.set_param("claim_filter", claim_filter)
-        if variant >= 0:
-            ctx.set_param("variant", variant)
-        if workers > 0:
-            ctx.set_param("workers", workers)
-        ctx.load(m)
-        s = ctx.find_start()
-        print(f"pmc probe: variant {variant}, {workers} workgroups (-1 / 0: library default)", flush=True)
-        for _ in range(runs):
-            ctx.run(s)
+    if "single" in legs:
+        with P.RcmContext(timeout_ms=30000) as ctx:
+            if claim_filter >= 0:
+                ctx.set_param("claim_filter", claim_filter)
+            if variant >= 0:
+                ctx.set_param("variant", variant)
+            if workers > 0:
+                ctx.set_param("workers", workers)
+            ctx.load(m)
+            s = ctx.find_start()
+            print(f"pmc probe: variant {variant}, {workers} workgroups (-1 / 0: library default)", flush=True)
+            for _ in range(runs):
+                ctx.run(s)
+    if "batched" in legs:
+        bs = P.workloads.batch_mix(4096, seed=4)
+        with P.RcmBatch(timeout_ms=120000, graph=False) as b:
+            b.set_param("variant", 4)
+            b.set_param("workers", 256)
+            b.set_param("helper_nodes", 131072)
+            b.load(bs)
+            starts, _, st = b.find_start()
+            print("pmc probe: batched mix of 4096, variant 4, 256 workgroups, helper nodes 131072", flush=True)
+            for _ in range(runs):
+                if not (b.run(starts) == 0).all():
+                    raise RuntimeError("batched run failed")
+    if "pipeline" in legs:
+        rng = np.random.default_rng(SEED_C2)
+        rows = np.repeat(np.arange(m.rows, dtype=np.int32), np.diff(m.row_offsets))
+        lower = rows >= m.col_ids
+        order = rng.permutation(int(lower.sum()))
+        r = np.ascontiguousarray(rows[lower][order])
+        c = np.ascontiguousarray(m.col_ids[lower][order])
+        v = rng.standard_normal(r.size)
+        print(f"pmc probe: canonicalisation of {r.size} triples", flush=True)
+        with P.MatrixOps() as ops:
+            for _ in range(runs):
+                ops.canonicalize(m.rows, m.rows, r, c, v, True)
     a = torch.empty(COPY_BYTES // 4, dtype=torch.int32, device="cuda")
     a.fill_(1)
     b = torch.empty_like(a)
@@ -469,6 +511,8 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=-1, help="--pmc-probe only: engine variant (-1: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", type=int, default=0, help="only the workload of the rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-legs", type=str, default="single",
+                    help="--pmc-probe only: comma list of single, batched, pipeline")
     ap.add_argument("--claim-filter", type=int, default=-1, help="claim pre-read filter 0/1/2 (-1: library default)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the default configuration instead of the autotuned one")
@@ -480,6 +524,8 @@ def main() -> int:
     ap.add_argument("--serving-fracs", type=str, default="0.25,0.5,0.75",
                     help="Poisson arrival rates as fractions of the measured capacity")
     ap.add_argument("--serving-lanes", type=int, default=4)
+    ap.add_argument("--serving-workers", type=int, default=0,
+                    help="persistent workgroups per serving lane (0: the library default)")
     ap.add_argument("--serving-cold-sample", type=int, default=48)
     ap.add_argument("--c3", type=int, default=1, help="the 1M-point random geometric graph leg (0 = skip)")
     ap.add_argument("--rmat", type=int, default=20, help="R-MAT scale of the engine-variant leg (0 = skip)")
@@ -487,7 +533,7 @@ def main() -> int:
                     help="steps of the canonicalise and permute legs on the 128^3 stencil (0 = skip)")
     args = ap.parse_args()
     if args.pmc_probe:
-        return pmc_probe(args.pmc_probe, args.claim_filter, args.variant, args.workers)
+        return pmc_probe(args.pmc_probe, args.claim_filter, args.variant, args.workers, args.pmc_legs)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "The batched form shards by matrix; see DESIGN.md.", file=sys.stderr)
@@ -510,20 +556,21 @@ def main() -> int:
     if args.claim_filter >= 0:
         ctx.set_param("claim_filter", args.claim_filter)
     ctx.load(m)
-    start = ctx.find_start()
-    ctx.run(start, P.RCM_FLAG_REVERSE)
+    start, _ = ctx.order(flags=P.RCM_FLAG_STABLE | P.RCM_FLAG_REVERSE)  # start search and order in one call
     bw = ctx.bandwidth()
     perm = ctx.perm()
     e2e_ms = (time.perf_counter() - e2e0) * 1e3  # includes creating the context (the process's first HIP calls)
     # the same steps again on the existing context
     w0 = time.perf_counter()
     ctx.load(m)
-    start = ctx.find_start()
-    ctx.run(start, P.RCM_FLAG_REVERSE)
+    start, _ = ctx.order(flags=P.RCM_FLAG_STABLE | P.RCM_FLAG_REVERSE)
     bw = ctx.bandwidth()
     perm = ctx.perm()
     e2e_warm_ms = (time.perf_counter() - w0) * 1e3
     tm = ctx.timing()
+    if ctx.find_start() != start:  # the two-call path's search alone, for comparison
+        raise RuntimeError("start search disagrees with the fused search")
+    search_ms = ctx.timing().start_search_ms
     assert P.checkPermutation(perm) == 0
 
     tune = None
@@ -571,7 +618,9 @@ def main() -> int:
         "init_kernel_ms": last.init_kernel_ms,
         "end_to_end_ms": e2e_ms,
         "end_to_end_warm_ms": e2e_warm_ms,
-        "start_search_ms": tm.start_search_ms,
+        "order_ms": tm.order_ms,
+        "order_passes": tm.order_passes,
+        "start_search_ms": search_ms,
         "start_search_iterations": tm.start_iterations,
         "load_ms": tm.load_ms,
         "bandwidth": bw,
@@ -601,7 +650,8 @@ def main() -> int:
     if args.serving > 0:
         fracs = [float(x) for x in args.serving_fracs.split(",") if x]
         print(f"[bench] serving form: {args.serving} requests", file=sys.stderr, flush=True)
-        line["serving"] = bench_serving(args.serving, fracs, 5, args.serving_lanes, args.serving_cold_sample)
+        line["serving"] = bench_serving(args.serving, fracs, 5, args.serving_lanes, args.serving_cold_sample,
+                                        args.serving_workers)
 
     if not args.no_cpu:
         print("[bench] CPU baseline (reference CPU paths on this host)", file=sys.stderr, flush=True)

@@ -83,11 +83,12 @@ enum rcm_param {
   RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
   RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width of the current variant (read-only: set the variant) */
   RCM_PARAM_VARIANT = 5,     /* engine variant (block size, batch width, radix digit):
-                                0 = 256 threads, 2048 adjacency slots, 256 parents, 4-bit digits (default)
+                                0 = 256 threads, 2048 adjacency slots, 256 parents, 4-bit digits (rcm_ctx
+                                    default; serving lanes)
                                 1 = 256 threads, 4096 slots, 256 parents, 4-bit digits
                                 2 = 128 threads, 1024 slots, 128 parents, 4-bit digits
                                 3 = 256 threads, 2048 slots, 256 parents, 6-bit digits
-                                4 = 512 threads, 4096 slots, 512 parents, 4-bit digits
+                                4 = 512 threads, 4096 slots, 512 parents, 4-bit digits (rcm_batch default)
                                 5 = level-synchronous engine (rcm_ctx only): one BFS level per step, children
                                     ordered by a device-wide radix sort with 8-bit digits; for wide, shallow
                                     graphs. The structure of GPU_SIMPLE (src/device/rcm_gpu.cu:134-330) with
@@ -101,14 +102,20 @@ enum rcm_param {
   RCM_PARAM_WG_MAX_N = 10  /* rcm_batch only, reserved: 0 is accepted, any other value is RCM_EINVAL (a
                               one-workgroup-per-matrix engine is not part of the library). */,
   RCM_PARAM_HELPER_NODES = 11 /* rcm_batch only: a matrix of n rows is helped by at most n / value + 1 persistent
-                                 workgroups at once (at most 1024); default 8192. Fewer helpers per matrix leave
-                                 fewer workgroups waiting on narrow levels (throughput of large sets); more give
-                                 each matrix a shorter time to result. rcm_batch_autotune may change it. */,
+                                 workgroups at once (at most 1024), or by its share of the launch's workgroups in
+                                 proportion to its n if that is more; default 131072 (2048 in serving lanes).
+                                 Fewer helpers per matrix leave fewer workgroups waiting on narrow levels
+                                 (throughput of large sets); more give each matrix a shorter time to result.
+                                 rcm_batch_autotune may change it. */,
   RCM_PARAM_BATCH_CAP = 12    /* rcm_ctx only: batch records a run may use; 0 (default) = n + 1, the most a run
                                  can need. A smaller value makes a run that needs more records stop with
                                  RCM_ECAPACITY (every workgroup exits; the context stays usable) -- the error
                                  path of the reference's oversize-batch __trap (include/device/batchWorker.cuh:
-                                 539-543), which cannot be reached otherwise. */
+                                 539-543), which cannot be reached otherwise. */,
+  RCM_PARAM_SPECULATE = 13    /* rcm_ctx_order / rcm_batch_order: from this start-search sweep on (default 3; 0 =
+                                 never), a sweep first runs as a CM run that also records BFS levels; when its
+                                 eccentricity equals the previous sweep's it is the confirming sweep and its
+                                 order the result, one engine pass fewer. Results are the same for any value. */
 };
 
 int rcm_ctx_create(int device, rcm_ctx** out);
@@ -126,6 +133,13 @@ int rcm_ctx_find_start(rcm_ctx* ctx, int32_t* start_out, int32_t* iterations_out
 
 /* Reorder the loaded matrix from start_node (>= 0). Result stays on the device. */
 int rcm_ctx_run(rcm_ctx* ctx, int32_t start_node, uint32_t flags);
+
+/* Start search and reorder in one call: start_node >= 0 is rcm_ctx_run; start_node < 0 gives the start, sweep count
+ * and order of rcm_ctx_find_start followed by rcm_ctx_run, in one engine pass fewer when the search stops at a
+ * speculated sweep (RCM_PARAM_SPECULATE). The device half of GPU_BATCH::rcm with its start from
+ * SimplePeripheral::findPeripheral (src/device/rcm_gpu_batch.cu:42-112, src/simple_peripheral.cpp:92-141).
+ * start_out, iterations_out may be NULL. */
+int rcm_ctx_order(rcm_ctx* ctx, int32_t start_node, uint32_t flags, int32_t* start_out, int32_t* iterations_out);
 
 /* Bandwidth of the loaded matrix under the last computed ordering (reversal does not change it). */
 int rcm_ctx_bandwidth(rcm_ctx* ctx, int64_t* bandwidth_out);
@@ -162,6 +176,8 @@ typedef struct rcm_timing {
   int32_t placed;           /* nodes placed by the last run */
   int32_t error_site;       /* device-side diagnostics of the last failed run (0 = none) */
   int32_t error_info[8];
+  double order_ms;          /* last rcm_ctx_order: start search and order, host wall time */
+  int32_t order_passes;     /* engine passes of the last rcm_ctx_order */
 } rcm_timing_t;
 int rcm_ctx_timing(rcm_ctx* ctx, rcm_timing_t* out);
 /* Per-phase profile of the last instrumented run (RCM_PARAM_PROFILE = 1): 100 MHz tick sums over all batches
@@ -194,6 +210,8 @@ typedef struct rcm_batch_timing {
   int64_t batches;         /* speculative batches of the last run, all matrices */
   int64_t total_n;         /* sum of n */
   int64_t total_nnz;       /* sum of nnz */
+  double order_ms;         /* last rcm_batch_order: start search and order, host wall time */
+  int32_t order_passes;    /* launches of the last rcm_batch_order */
 } rcm_batch_timing_t;
 
 int rcm_batch_create(int device, rcm_batch** out);
@@ -212,6 +230,12 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
 int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_out, int32_t* status_out);
 /* Reorder every matrix from start[k] (>= 0). status_out[k] (nullable): RCM_OK, RCM_ENOTCONNECTED, ... */
 int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* status_out);
+/* Start search and reorder of every matrix in one call: matrix k is ordered from start_in[k] when start_in is not
+ * NULL and start_in[k] >= 0, else from its rcm_batch_find_start start. Same starts, sweep counts, orders and
+ * statuses as rcm_batch_find_start + rcm_batch_run, with one engine pass fewer for every matrix whose search stops
+ * at a speculated sweep (RCM_PARAM_SPECULATE). Output arrays may be NULL. */
+int rcm_batch_order(rcm_batch* b, const int32_t* start_in, uint32_t flags, int32_t* start_out,
+                    int32_t* iterations_out, int32_t* status_out);
 /* Bandwidth of every matrix under the last run's order. */
 int rcm_batch_bandwidth(rcm_batch* b, int64_t* bandwidth_out);
 /* Copy every order to the host: matrix k's n[k] entries at perm_cat + po_off[k] (po_off nullable: back to
@@ -235,7 +259,9 @@ typedef struct rcm_server_config {
   int32_t slots;           /* requests per group at most (1..4096); 0 = 256 */
   int32_t lanes;           /* groups on the GPU at once (1..16); 0 = 4 */
   int64_t arena_bytes;     /* device memory for all lanes; 0 = half of the free HBM, at most 64 GiB */
-  int32_t workers;         /* persistent workgroups per lane; 0 = the co-resident capacity / lanes */
+  int32_t workers;         /* persistent workgroups per lane; 0 = 2 x (co-resident capacity / lanes), at most the
+                              capacity: a lane's idle workgroups leave its launch, so lanes with little work
+                              yield their slots to the others */
   int64_t job_timeout_ms;  /* device watchdog of a launch; 0 = 60000 */
 } rcm_server_config;
 
@@ -260,6 +286,11 @@ typedef struct rcm_server_stats {
   int64_t loop_iterations;                /* groups formed */
   int64_t probe_rounds;                   /* unused (0) */
   int64_t claims;                         /* unused (0) */
+  /* Where a group's time goes, summed over all groups (host wall time, ms): concatenating the requests' CSRs on
+   * the host; rcm_batch_load (H2D copies and validation); rcm_batch_order (start search and order); bandwidth and
+   * D2H copy of the orders; splitting the results per request. */
+  double group_host_copy_ms, group_load_ms, group_order_ms, group_readback_ms, group_deliver_ms;
+  int64_t group_launches;                 /* engine launches of all groups */
 } rcm_server_stats;
 
 int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out);

@@ -70,6 +70,7 @@ PARAM_CLAIM_FILTER = 9
 PARAM_WG_MAX_N = 10
 PARAM_HELPER_NODES = 11
 PARAM_BATCH_CAP = 12
+PARAM_SPECULATE = 13
 PROFILE_PHASES = ("acquire", "parents", "claim", "finalise", "sort", "cut", "prefix", "write")
 
 
@@ -519,6 +520,8 @@ class Timing:
     placed: int
     error_site: int = 0
     error_info: tuple = ()
+    order_ms: float = 0.0
+    order_passes: int = 0
 
 
 class RcmContext:
@@ -535,7 +538,8 @@ class RcmContext:
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT,
                "digit_bits": PARAM_DIGIT_BITS, "variant": PARAM_VARIANT, "strict": PARAM_STRICT,
-               "profile": PARAM_PROFILE, "claim_filter": PARAM_CLAIM_FILTER, "batch_cap": PARAM_BATCH_CAP}
+               "profile": PARAM_PROFILE, "claim_filter": PARAM_CLAIM_FILTER, "batch_cap": PARAM_BATCH_CAP,
+               "speculate": PARAM_SPECULATE}
 
     def set_param(self, name: str, value: int) -> None:
         _check(self._lib.rcm_ctx_set_param(self._h, self._PARAMS[name], int(value)), f"set_param {name}")
@@ -587,6 +591,15 @@ class RcmContext:
     def run_status(self, start: int, flags: int = RCM_FLAG_STABLE) -> int:
         return int(self._lib.rcm_ctx_run(self._h, int(start), flags))
 
+    def order(self, start: int = -1, flags: int = RCM_FLAG_STABLE):
+        """Start search (when start < 0) and reorder in one call; returns (start used, sweeps)."""
+        s, it = C.c_int32(-1), C.c_int32(0)
+        rc = self._lib.rcm_ctx_order(self._h, int(start), flags, C.byref(s), C.byref(it))
+        if rc != RCM_OK:
+            t = self.timing()
+            raise RcmError(rc, f"order (device diagnostics: site {t.error_site}, {t.error_info})")
+        return int(s.value), int(it.value)
+
     def bandwidth(self) -> int:
         bw = C.c_int64()
         _check(self._lib.rcm_ctx_bandwidth(self._h, C.byref(bw)), "bandwidth")
@@ -630,7 +643,8 @@ class RcmContext:
         t = TimingT()
         _check(self._lib.rcm_ctx_timing(self._h, C.byref(t)), "timing")
         return Timing(t.run_kernel_ms, t.init_kernel_ms, t.start_search_ms, t.bandwidth_ms, t.load_ms, t.workers,
-                      t.batches, t.start_iterations, t.placed, t.error_site, tuple(t.error_info))
+                      t.batches, t.start_iterations, t.placed, t.error_site, tuple(t.error_info), t.order_ms,
+                      t.order_passes)
 
 
 def rcm_gpu_batch(m: CSR, start_node: int = -1, flags: int = RCM_FLAG_STABLE):
@@ -675,7 +689,7 @@ class RcmBatch:
 
     _PARAMS = {"workers": PARAM_WORKERS, "timeout_ms": PARAM_TIMEOUT_MS, "sort": PARAM_SORT, "graph": PARAM_GRAPH,
                "claim_filter": PARAM_CLAIM_FILTER, "variant": PARAM_VARIANT, "wg_max_n": PARAM_WG_MAX_N,
-               "helper_nodes": PARAM_HELPER_NODES}
+               "helper_nodes": PARAM_HELPER_NODES, "speculate": PARAM_SPECULATE}
 
     def autotune(self, starts, repeats: int = 1) -> TuneResultT:
         """Time every engine variant x workgroup count on the loaded set and keep the fastest."""
@@ -711,6 +725,18 @@ class RcmBatch:
         if rc in (RCM_EDEVICE, RCM_ETIMEOUT, RCM_ENOTLOADED, RCM_EINVAL):
             raise RcmError(rc, "rcm_batch_run")
         return st
+
+    def order(self, starts=None, flags: int = RCM_FLAG_STABLE):
+        """Start search and reorder of every matrix in one call (starts[k] >= 0 skips the search for matrix k).
+        Returns (starts used, sweeps, per-matrix status); device-level failures raise."""
+        k = len(self._bs)
+        s, it, st = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(k, np.int32)
+        sarr = None if starts is None else _i32(starts)
+        rc = _native.lib().rcm_batch_order(self._h, None if sarr is None else _p32(sarr), flags, _p32(s), _p32(it),
+                                           _p32(st))
+        if rc in (RCM_EDEVICE, RCM_ETIMEOUT, RCM_ENOTLOADED, RCM_EINVAL):
+            raise RcmError(rc, "rcm_batch_order")
+        return s, it, st
 
     def run_status(self, starts, flags: int = RCM_FLAG_STABLE):
         """Reorder every matrix; returns (return code, per-matrix status) without raising."""

@@ -45,6 +45,8 @@ class TimingT(C.Structure):
         ("placed", C.c_int32),
         ("error_site", C.c_int32),
         ("error_info", C.c_int32 * 8),
+        ("order_ms", C.c_double),
+        ("order_passes", C.c_int32),
     ]
 
 
@@ -61,6 +63,8 @@ class BatchTimingT(C.Structure):
         ("batches", C.c_int64),
         ("total_n", C.c_int64),
         ("total_nnz", C.c_int64),
+        ("order_ms", C.c_double),
+        ("order_passes", C.c_int32),
     ]
 
 
@@ -104,6 +108,12 @@ class ServerStatsT(C.Structure):
         ("loop_iterations", C.c_int64),
         ("probe_rounds", C.c_int64),
         ("claims", C.c_int64),
+        ("group_host_copy_ms", C.c_double),
+        ("group_load_ms", C.c_double),
+        ("group_order_ms", C.c_double),
+        ("group_readback_ms", C.c_double),
+        ("group_deliver_ms", C.c_double),
+        ("group_launches", C.c_int64),
     ]
 
 
@@ -136,6 +146,7 @@ _SIGNATURES = {
     "rcm_ctx_load_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p]),
     "rcm_ctx_find_start": (C.c_int, [C.c_void_p, I32P, I32P]),
     "rcm_ctx_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32]),
+    "rcm_ctx_order": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, I32P, I32P]),
     "rcm_ctx_bandwidth": (C.c_int, [C.c_void_p, I64P]),
     "rcm_ctx_bandwidth_of": (C.c_int, [C.c_void_p, I32P, I64P]),
     "rcm_ctx_read_perm": (C.c_int, [C.c_void_p, I32P]),
@@ -148,6 +159,7 @@ _SIGNATURES = {
     "rcm_batch_load": (C.c_int, [C.c_void_p, C.c_int32, I32P, I64P, I64P, I64P, I32P, I32P, I32P]),
     "rcm_batch_find_start": (C.c_int, [C.c_void_p, I32P, I32P, I32P]),
     "rcm_batch_run": (C.c_int, [C.c_void_p, I32P, C.c_uint32, I32P]),
+    "rcm_batch_order": (C.c_int, [C.c_void_p, I32P, C.c_uint32, I32P, I32P, I32P]),
     "rcm_batch_bandwidth": (C.c_int, [C.c_void_p, I64P]),
     "rcm_batch_read_perm": (C.c_int, [C.c_void_p, I64P, I32P]),
     "rcm_batch_perm_device": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),

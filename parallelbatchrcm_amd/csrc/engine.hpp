@@ -74,7 +74,7 @@ struct RunArgs {
   uint64_t* claim;        // per node claim code
   uint64_t* bstat;        // per batch status word
   uint64_t* brange;       // per batch: begin | end << 32 (0 = not yet created)
-  int32_t* level;         // per queue position BFS level (BFS mode only)
+  int32_t* level;         // per queue position BFS level (BFS runs, and CM runs of the kernels built to record them)
   Control* ctl;
   int32_t n;
   int32_t bcap;           // records in bstat / brange
@@ -420,7 +420,8 @@ __device__ __forceinline__ uint32_t row_degree(const RunArgs& a, int32_t c) {
   return (uint32_t)(a.row_ptr[c + 1] - a.row_ptr[c]);
 }
 
-template <class G, int DIGIT, bool STRICT, bool BFS>
+// LEV: a CM run that also records BFS levels (the start search's speculated sweep); BFS runs always do.
+template <class G, int DIGIT, bool STRICT, bool BFS, bool LEV = false>
 __device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_t b, int32_t s, int32_t parent,
                                    uint32_t rb, uint32_t D, int32_t plevel, bool early, uint64_t deadline) {
   constexpr int T = G::T, S = G::S, EPT = G::EPT, W = G::W;
@@ -636,6 +637,7 @@ __device__ void process_long_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint32_
     for (uint32_t o = (uint32_t)tid; o < cnt; o += T) {
       const int32_t c = (int32_t)ld_sync(&bufA[o]);
       st_sync(&a.qinfo[xo + o], (uint64_t)(uint32_t)a.row_ptr[c] | ((uint64_t)(uint32_t)a.row_ptr[c + 1] << 32));
+      if constexpr (LEV) st_sync(&a.level[xo + o], plevel + 1);
     }
   }
   publish_wait<STRICT>();
@@ -708,7 +710,7 @@ __device__ __forceinline__ void window_verdict(SM& sm, uint64_t w) {
 }
 
 // ---- one batch ------------------------------------------------------------------------------------------
-template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF>
+template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF, bool LEV = false>
 __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t deadline, uint64_t* pf) {
   constexpr int T = G::T, S = G::S, EPT = G::EPT, W = G::W;
   const int tid = threadIdx.x;
@@ -741,12 +743,12 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       const uint64_t q = ld_sync(&a.qinfo[s + tid]);
       rb = (uint32_t)q;
       re = (uint32_t)(q >> 32);
-      if constexpr (BFS) lev = ld_sync(&a.level[s + tid]);
+      if constexpr (BFS || LEV) lev = ld_sync(&a.level[s + tid]);
     }
     sm.pnode[tid] = p;
     sm.prb[tid] = rb;
     d = re - rb;
-    if constexpr (BFS) sm.plevel[tid] = lev;
+    if constexpr (BFS || LEV) sm.plevel[tid] = lev;
   }
   if (wave_id() == 0) {
     const bool ok = kept ? sm.kearly != 0 : claims_complete_before(a, (int64_t)b);
@@ -784,8 +786,9 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
       stop_workgroup(sm);
       return;
     }
-    process_long_batch<G, DIGIT, STRICT, BFS>(a, sm, b, s, sm.pnode[0], sm.prb[0], D, BFS ? sm.plevel[0] : 0,
-                                              sm.early != 0, deadline);
+    const int32_t plev = (BFS || LEV) ? sm.plevel[0] : 0;
+    process_long_batch<G, DIGIT, STRICT, BFS, LEV>(a, sm, b, s, sm.pnode[0], sm.prb[0], D, plev, sm.early != 0,
+                                                   deadline);
     return;
   }
 
@@ -1139,10 +1142,10 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
     const uint32_t p = xo + o;
     const int32_t child = sm.sc[j];
     const uint32_t crb = sm.srb[j], cre = sm.sre[j];
-    const int32_t clev = BFS ? sm.plevel[sm.si[j]] + 1 : 0;
+    const int32_t clev = (BFS || LEV) ? sm.plevel[sm.si[j]] + 1 : 0;
     st_sync(&a.perm[p], child);
     st_sync(&a.qinfo[p], (uint64_t)crb | ((uint64_t)cre << 32));
-    if constexpr (BFS) st_sync(&a.level[p], clev);
+    if constexpr (BFS || LEV) st_sync(&a.level[p], clev);
     if (keep && o < kend) {
       sm.key[fin][o] = (uint64_t)(uint32_t)child | ((uint64_t)crb << 32);
       sm.key[fin][G::P + o] = (uint64_t)cre | ((uint64_t)(uint32_t)clev << 32);
@@ -1197,7 +1200,7 @@ __device__ void process_batch(const RunArgs& a, Smem<G, DIGIT>& sm, uint64_t dea
 }
 
 // ---- the persistent kernel --------------------------------------------------------------------------------
-template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF = false>
+template <class G, int DIGIT, bool STRICT, bool BFS, bool PROF = false, bool LEV = false>
 __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
   using SmemT = Smem<G, DIGIT>;
   RCM_LDS(SmemT, sm);
@@ -1250,7 +1253,7 @@ __global__ __launch_bounds__(G::T) void k_reorder(RunArgs a) {
       if (tid == 0) pf[15] = wall_ticks();
     }
     consume_fence<STRICT>();
-    process_batch<G, DIGIT, STRICT, BFS, PROF>(a, sm, deadline, pf);
+    process_batch<G, DIGIT, STRICT, BFS, PROF, LEV>(a, sm, deadline, pf);
     __syncthreads();
     const int quit = sm.quit;
     __syncthreads();  // every wave has read the scalars before thread 0 rewrites them for the next batch

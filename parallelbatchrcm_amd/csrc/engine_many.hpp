@@ -14,7 +14,8 @@
 //     begins the next job of the launch's order (largest first, so long chains start early); otherwise it
 //     joins any job below its cap; otherwise it backs off.
 //   * Completion. The first mark of a job's `done` flag (complete or failed) increments `jobs_done`; the
-//     launch ends when it reaches the job count or the watchdog sets `abort`.
+//     launch ends when it reaches the job count or the watchdog sets `abort`. A free workgroup leaves earlier,
+//     once every job has begun and it has seen each one done or at its cap (neither can change back).
 #pragma once
 #include "engine.hpp"
 
@@ -54,7 +55,18 @@ struct ManyArgs {
   int32_t sort_mode;
   int32_t seg_threshold;
   int32_t claim_filter;
+  // A job of n rows may also take share_workers * n / share_rows helpers (its share of the launch's workgroups by
+  // size; 0 = none): a launch over a few matrices, such as the last sweeps of a start search, then spreads its
+  // workgroups over them instead of leaving most idle behind caps sized for thousands of matrices.
+  uint32_t share_workers;
+  uint64_t share_rows;
 };
+
+__device__ __forceinline__ uint32_t job_cap(const ManyArgs& m, const JobDesc& J) {
+  uint32_t c = (uint32_t)J.cap;
+  if (m.share_rows) c = max(c, (uint32_t)min((uint64_t)1024, (uint64_t)m.share_workers * (uint64_t)J.n / m.share_rows));
+  return c;
+}
 
 template <class G, int DIGIT>
 struct ManySmem {
@@ -63,7 +75,7 @@ struct ManySmem {
   uint32_t cursor;
 };
 
-__device__ __forceinline__ RunArgs job_args(const ManyArgs& m, int32_t j, bool bfs) {
+__device__ __forceinline__ RunArgs job_args(const ManyArgs& m, int32_t j, bool levels) {
   const JobDesc& J = m.jobs[j];
   RunArgs a{};
   a.row_ptr = J.row_ptr;
@@ -73,7 +85,7 @@ __device__ __forceinline__ RunArgs job_args(const ManyArgs& m, int32_t j, bool b
   a.claim = J.claim;
   a.bstat = J.bstat;
   a.brange = J.brange;
-  a.level = bfs ? J.level : nullptr;
+  a.level = levels ? J.level : nullptr;
   a.ctl = J.ctl;
   a.n = J.n;
   a.bcap = J.n + 1;
@@ -95,7 +107,8 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
   int32_t cur = ms.cur;
   uint32_t cursor = ms.cursor;
   const uint32_t count = (uint32_t)m.count;
-  uint32_t idle = 0;  // consecutive rounds without work: backoff grows with it
+  uint32_t idle = 0;    // consecutive rounds without work: backoff grows with it
+  uint32_t closed = 0;  // jobs seen closed (done or at their cap) in a row since every job was begun
   for (;;) {
     const uint32_t fin = wave_uniform(ld_sync(&m.g->jobs_done));
     const uint32_t ab = wave_uniform(ld_sync(&m.g->abort));
@@ -162,13 +175,26 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
     if (lane >= 1 && (uint32_t)lane <= begun) {  // each begun job at most once per round
       j = m.order[(cursor + (uint32_t)lane - 1u) % begun];
       const JobDesc& J = m.jobs[j];
-      if (!ld_sync(&J.ctl->done) && ld_sync(&m.helpers[j]) < (uint32_t)J.cap) {
+      if (!ld_sync(&J.ctl->done) && ld_sync(&m.helpers[j]) < job_cap(m, J)) {
         open = true;
         const uint32_t t = ld_sync(&J.ctl->ticket);
         ready = t <= (uint32_t)J.n && ld_sync(&J.brange[t]) != 0;
       }
     }
     cursor += kWave - 1;
+    // Once every job has begun, a closed job never opens again: helpers only join below the cap and leave when
+    // the job is done. A free workgroup that has seen every job closed can never get work, so it leaves the
+    // launch (freeing its slot for other launches) instead of polling until the last job finishes.
+    if (begun >= count) {
+      closed = ballot(open) ? 0u : closed + min(begun, (uint32_t)(kWave - 1));
+      if (closed >= count) {
+        if (lane == 0) {
+          ms.cur = -1;
+          sm.quit = 1;
+        }
+        return;
+      }
+    }
     uint64_t cand = ballot(ready);
     if (!cand && begun < count) {
       uint32_t idx = 0;
@@ -188,7 +214,7 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
       uint32_t ok = 0;
       if (lane == l) {
         const uint32_t was = atomic_add_u32(&m.helpers[j], 1u);
-        ok = was < (uint32_t)m.jobs[j].cap ? 1u : 0u;
+        ok = was < job_cap(m, m.jobs[j]) ? 1u : 0u;
         if (!ok) atomic_add_u32(&m.helpers[j], 0xFFFFFFFFu);  // full meanwhile
       }
       joined = ballot(ok != 0) != 0;
@@ -205,7 +231,8 @@ __device__ void find_work(const ManyArgs& m, ManySmem<G, DIGIT>& ms, uint64_t de
   }
 }
 
-template <class G, int DIGIT, bool STRICT, bool BFS>
+// LEV: CM runs that also record BFS levels (the start search's speculated sweeps).
+template <class G, int DIGIT, bool STRICT, bool BFS, bool LEV = false>
 __global__ __launch_bounds__(G::T) void k_reorder_many(ManyArgs m) {
   using MS = ManySmem<G, DIGIT>;
   RCM_LDS(MS, ms);
@@ -225,8 +252,8 @@ __global__ __launch_bounds__(G::T) void k_reorder_many(ManyArgs m) {
       if (ms.s.quit) break;
     }
     consume_fence<STRICT>();
-    const RunArgs a = job_args(m, ms.cur, BFS);
-    process_batch<G, DIGIT, STRICT, BFS, false>(a, ms.s, deadline, nullptr);
+    const RunArgs a = job_args(m, ms.cur, BFS || LEV);
+    process_batch<G, DIGIT, STRICT, BFS, false, LEV>(a, ms.s, deadline, nullptr);
     __syncthreads();  // every wave is done with the batch before wave 0 overwrites its scalars
   }
 }

@@ -30,7 +30,16 @@ using namespace rcm;
 namespace {
 
 constexpr int kDefaultSegThreshold = 24;
-constexpr int64_t kDefaultHelperNodes = 8192;  // a matrix of n rows: at most n / 8192 + 1 helpers
+constexpr int64_t kDefaultHelperNodes = 8192;  // the autotuner's helper caps: n / 8192, / 32768, / 131072 + 1
+// rcm_batch defaults: what rcm_batch_autotune picks on the 4096-matrix mix (variant 4, one helper per 131072
+// rows: 117 ms per launch against 172 ms for variant 0 with one per 8192). A launch over a few matrices still
+// spreads its workgroups over them by size (ManyArgs::share_rows), so the large quantum does not starve them.
+constexpr int64_t kDefaultBatchHelperNodes = 131072;
+constexpr int64_t kDefaultBatchVariant = 4;
+// RCM_PARAM_SPECULATE: from this start-search sweep on, the sweep runs as a CM run that records levels. On the
+// 4096-matrix mix 98% of the matrices stop at sweep 3, 1.8% at 4 and 0.4% at 2, so speculating from sweep 3
+// turns (sweeps + 1) engine passes into sweeps for almost every matrix.
+constexpr int64_t kDefaultSpeculate = 3;
 // Serving lanes give each request more helpers: latency, not throughput, is the aim there. Measured on MI355X
 // (Poisson stream, 95% mixed sizes and 5% R-MAT): 2048 against 8192 halves p50 at 200 and 400 requests/s.
 constexpr int64_t kServerHelperNodes = 2048;
@@ -65,39 +74,43 @@ constexpr int kLevelDigit = 8;
 using GeoSmall = Var<0>::G;  // the many-matrix helpers' default
 constexpr int kDigit = 4;
 
+// lev: a CM run that also records BFS levels (a speculated start-search sweep); plain-BFS runs always do.
 template <int V>
-void* reorder_kernel_v(bool strict, bool bfs, bool prof) {
+void* reorder_kernel_v(bool strict, bool bfs, bool prof, bool lev) {
   using G = typename Var<V>::G;
   constexpr int D = Var<V>::D;
   if (strict) return bfs ? reinterpret_cast<void*>(&k_reorder<G, D, true, true>)
-                         : reinterpret_cast<void*>(&k_reorder<G, D, true, false>);
+                         : lev ? reinterpret_cast<void*>(&k_reorder<G, D, true, false, false, true>)
+                               : reinterpret_cast<void*>(&k_reorder<G, D, true, false>);
   if (bfs) return reinterpret_cast<void*>(&k_reorder<G, D, false, true>);
+  if (lev) return reinterpret_cast<void*>(&k_reorder<G, D, false, false, false, true>);
   return prof ? reinterpret_cast<void*>(&k_reorder<G, D, false, false, true>)
               : reinterpret_cast<void*>(&k_reorder<G, D, false, false>);
 }
-void* reorder_kernel(int v, bool strict, bool bfs, bool prof) {
+void* reorder_kernel(int v, bool strict, bool bfs, bool prof, bool lev = false) {
   switch (v) {
-    case 1: return reorder_kernel_v<1>(strict, bfs, prof);
-    case 2: return reorder_kernel_v<2>(strict, bfs, prof);
-    case 3: return reorder_kernel_v<3>(strict, bfs, prof);
-    case 4: return reorder_kernel_v<4>(strict, bfs, prof);
-    default: return reorder_kernel_v<0>(strict, bfs, prof);
+    case 1: return reorder_kernel_v<1>(strict, bfs, prof, lev);
+    case 2: return reorder_kernel_v<2>(strict, bfs, prof, lev);
+    case 3: return reorder_kernel_v<3>(strict, bfs, prof, lev);
+    case 4: return reorder_kernel_v<4>(strict, bfs, prof, lev);
+    default: return reorder_kernel_v<0>(strict, bfs, prof, lev);
   }
 }
 template <int V>
-void* many_kernel_v(bool bfs) {
+void* many_kernel_v(bool bfs, bool lev) {
   using G = typename Var<V>::G;
   constexpr int D = Var<V>::D;
-  return bfs ? reinterpret_cast<void*>(&k_reorder_many<G, D, false, true>)
+  if (bfs) return reinterpret_cast<void*>(&k_reorder_many<G, D, false, true>);
+  return lev ? reinterpret_cast<void*>(&k_reorder_many<G, D, false, false, true>)
              : reinterpret_cast<void*>(&k_reorder_many<G, D, false, false>);
 }
-void* many_kernel(int v, bool bfs) {
+void* many_kernel(int v, bool bfs, bool lev = false) {
   switch (v) {
-    case 1: return many_kernel_v<1>(bfs);
-    case 2: return many_kernel_v<2>(bfs);
-    case 3: return many_kernel_v<3>(bfs);
-    case 4: return many_kernel_v<4>(bfs);
-    default: return many_kernel_v<0>(bfs);
+    case 1: return many_kernel_v<1>(bfs, lev);
+    case 2: return many_kernel_v<2>(bfs, lev);
+    case 3: return many_kernel_v<3>(bfs, lev);
+    case 4: return many_kernel_v<4>(bfs, lev);
+    default: return many_kernel_v<0>(bfs, lev);
   }
 }
 
@@ -150,6 +163,7 @@ struct rcm_ctx {
   int64_t strict = 0;
   int64_t profile = 0;
   int64_t batch_cap = 0;     // RCM_PARAM_BATCH_CAP: batch records a run may use (0 = n + 1, all it can need)
+  int64_t speculate = kDefaultSpeculate;  // RCM_PARAM_SPECULATE
   uint64_t* prof = nullptr;  // kProfSlots words
   uint64_t* trace = nullptr;  // RCM_PARAM_PROFILE = 2: kTraceWords words per batch id
   int64_t trace_cap = 0;
@@ -253,8 +267,9 @@ int run_level(rcm_ctx* c, int32_t start, bool bfs) {
   return RCM_OK;
 }
 
-// One engine run from `start` (CM order, or plain BFS order with levels when bfs).
-int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
+// One engine run from `start` (CM order, or plain BFS order with levels when bfs; a CM run also records the
+// levels when `levels`).
+int run_engine(rcm_ctx* c, int32_t start, bool bfs, bool levels = false) {
   if (!c->rp) return RCM_ENOTLOADED;
   if (start < 0 || start >= c->n) return RCM_EINVAL;
   if (c->variant == kLevelVariant) return run_level(c, start, bfs);
@@ -266,7 +281,7 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   a.claim = c->claim;
   a.bstat = c->bstat;
   a.brange = c->brange;
-  a.level = bfs ? c->level : nullptr;
+  a.level = (bfs || levels) ? c->level : nullptr;
   a.ctl = c->ctl;
   a.n = c->n;
   a.bcap = (c->batch_cap > 0 && c->batch_cap < (int64_t)c->n + 1) ? (int32_t)c->batch_cap : c->n + 1;
@@ -274,7 +289,7 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   a.sort_mode = (int32_t)c->sort_mode;
   a.claim_filter = (int32_t)c->claim_filter;
   a.seg_threshold = kDefaultSegThreshold;
-  const bool prof = c->profile && !bfs && !c->strict;
+  const bool prof = c->profile && !bfs && !c->strict && !levels;
   a.prof = prof ? c->prof : nullptr;
   if (prof) RCM_HIP(hipMemsetAsync(c->prof, 0, kProfSlots * sizeof(uint64_t), c->stream));
   if (prof && c->profile == 2) {  // per-batch trace
@@ -296,7 +311,7 @@ int run_engine(rcm_ctx* c, int32_t start, bool bfs) {
   RCM_HIP(hipEventRecord(c->ev[0], c->stream));
   hipLaunchKernelGGL(k_init, dim3(init_blocks), dim3(256), 0, c->stream, a, start, bclear);
   RCM_HIP(hipEventRecord(c->ev[1], c->stream));
-  void* kern = reorder_kernel((int)c->variant, c->strict != 0, bfs, prof);
+  void* kern = reorder_kernel((int)c->variant, c->strict != 0, bfs, prof, levels && !bfs);
   void* args[] = {&a};
   RCM_HIP(hipLaunchKernel(kern, dim3(workers), dim3(kVarThreads[c->variant]), args, 0, c->stream));
   RCM_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -377,6 +392,17 @@ int verify_device(rcm_ctx* c, const int32_t* d_perm, int32_t start, bool bfs, in
   return RCM_OK;
 }
 
+// After the engine placed the CM order in perm: the reversed order for RCM_FLAG_REVERSE, and the result flags.
+int finish_order(rcm_ctx* c, uint32_t flags) {
+  c->reversed = (flags & RCM_FLAG_REVERSE) != 0;
+  if (c->reversed) {
+    hipLaunchKernelGGL(k_reverse, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->perm, c->n, c->out);
+    RCM_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->have_perm = true;
+  return RCM_OK;
+}
+
 int load_common(rcm_ctx* c, int32_t n, int64_t nnz) {
   int rc = ensure_state(c, n);
   if (rc) return rc;
@@ -409,9 +435,10 @@ struct rcm_batch {
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
   int64_t use_graph = 1;
-  int64_t helper_nodes = kDefaultHelperNodes;  // RCM_PARAM_HELPER_NODES
+  int64_t helper_nodes = kDefaultBatchHelperNodes;  // RCM_PARAM_HELPER_NODES
+  int64_t speculate = kDefaultSpeculate;        // RCM_PARAM_SPECULATE
   std::vector<JobDesc> jobs_host;              // host copy of the loaded set's job descriptors
-  int64_t variant = 0;
+  int64_t variant = kDefaultBatchVariant;
   int max_resident = 0;              // of the current variant
   int max_res[kVariants] = {};       // co-resident workgroups of each variant
   // the loaded set
@@ -582,11 +609,15 @@ int batch_workers(rcm_batch* b) {
 
 // One pass over a list of matrices (d_list, count entries; d_blk is their block map of nblk blocks): the reset
 // and the persistent many-matrix launch.
-void enqueue_many(rcm_batch* b, const int32_t* d_list, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs) {
+// `rows`: the listed matrices' total n, over which the launch's workgroups are shared by size (ManyArgs).
+void enqueue_many(rcm_batch* b, const int32_t* d_list, const int32_t* d_blk, int32_t nblk, int32_t count, bool bfs,
+                  bool levels, int64_t rows) {
   if (count <= 0) return;
-  const ManyArgs m = batch_args(b, d_list, count);
+  ManyArgs m = batch_args(b, d_list, count);
+  m.share_workers = (uint32_t)batch_workers(b);
+  m.share_rows = (uint64_t)std::max<int64_t>(rows, 0);
   hipLaunchKernelGGL(k_init_many, dim3((unsigned)nblk), dim3(256), 0, b->stream, m, d_blk);
-  void* kern = many_kernel((int)b->variant, bfs);
+  void* kern = many_kernel((int)b->variant, bfs, levels && !bfs);
   ManyArgs mm = m;
   void* args[] = {&mm};
   (void)hipLaunchKernel(kern, dim3(batch_workers(b)), dim3(kVarThreads[b->variant]), args, 0, b->stream);
@@ -687,6 +718,9 @@ struct rcm_server {
   int64_t next_id = 1;
   int64_t inflight_bytes = 0;
   rcm_server_stats st{};
+  // per lane, grow-only host staging for a group's concatenated CSRs and orders: reused, so a group does not
+  // pay for fresh zero-filled pages
+  std::vector<std::vector<int32_t>> stage_rp, stage_ci, stage_perm;
 };
 
 namespace {
@@ -695,10 +729,14 @@ namespace {
 void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>& grp) {
   rcm_batch* b = s->batch[(size_t)lane];
   const int32_t C = (int32_t)grp.size();
+  using clk = std::chrono::steady_clock;
+  const auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+  auto t = clk::now();
+  double copy_ms = 0, load_ms = 0, order_ms = 0, read_ms = 0, deliver_ms = 0;
   std::vector<int32_t> n((size_t)C), starts((size_t)C, 0), st((size_t)C, RCM_OK), st2((size_t)C, RCM_OK);
   std::vector<int64_t> nnz((size_t)C), rp_off((size_t)C), ci_off((size_t)C);
   int64_t rpn = 0, cin = 0;
-  bool search = false, want_bw = false;
+  bool want_bw = false;
   for (int32_t k = 0; k < C; ++k) {
     n[k] = grp[k]->n;
     nnz[k] = grp[k]->nnz;
@@ -706,16 +744,21 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     ci_off[k] = cin;
     rpn += n[k] + 1;
     cin += nnz[k];
-    search = search || grp[k]->start_req < 0;
     want_bw = want_bw || (grp[k]->flags & RCM_FLAG_BANDWIDTH);
   }
-  std::vector<int32_t> rp_cat((size_t)rpn), ci_cat((size_t)std::max<int64_t>(cin, 1));
+  std::vector<int32_t>& rp_cat = s->stage_rp[(size_t)lane];
+  std::vector<int32_t>& ci_cat = s->stage_ci[(size_t)lane];
+  if (rp_cat.size() < (size_t)rpn) rp_cat.resize((size_t)rpn);
+  if (ci_cat.size() < (size_t)std::max<int64_t>(cin, 1)) ci_cat.resize((size_t)std::max<int64_t>(cin, 1));
   for (int32_t k = 0; k < C; ++k) {
     std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat.begin() + rp_off[k]);
     std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat.begin() + ci_off[k]);
   }
+  copy_ms = ms_since(t);
+  t = clk::now();
   int hard = rcm_batch_load(b, C, n.data(), nnz.data(), rp_off.data(), ci_off.data(), rp_cat.data(), ci_cat.data(),
                             st.data());
+  load_ms = ms_since(t);
   std::vector<int32_t> iters((size_t)C, 0);
   if (hard == RCM_EINVAL) {  // some request is malformed: report it, run the rest without it
     std::vector<std::shared_ptr<ServerJob>> good, bad;
@@ -729,27 +772,25 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     grp.insert(grp.end(), bad.begin(), bad.end());
     return;
   }
-  if (!hard && search) {
-    const int r = rcm_batch_find_start(b, starts.data(), iters.data(), st.data());
-    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT || r == RCM_ENOTLOADED) hard = r;
-  }
-  for (int32_t k = 0; k < C; ++k)
-    if (grp[k]->start_req >= 0) {
-      starts[k] = grp[k]->start_req;
-      st[k] = RCM_OK;
-      iters[k] = 0;
-    }
-  if (!hard) {
-    const int r = rcm_batch_run(b, starts.data(), RCM_FLAG_STABLE, st2.data());
+  t = clk::now();
+  int launches = 0;
+  if (!hard) {  // start search (for requests without a start) and order, in one call
+    for (int32_t k = 0; k < C; ++k) starts[k] = grp[k]->start_req;
+    const int r = rcm_batch_order(b, starts.data(), RCM_FLAG_STABLE, starts.data(), iters.data(), st2.data());
     if (r == RCM_EDEVICE || r == RCM_ETIMEOUT || r == RCM_ENOTLOADED || r == RCM_EINVAL) hard = r;
+    launches = b->timing.order_passes;
   }
+  order_ms = ms_since(t);
+  t = clk::now();
   std::vector<int64_t> bw((size_t)C, -1);
-  std::vector<int32_t> cat;
+  std::vector<int32_t>& cat = s->stage_perm[(size_t)lane];
   if (!hard && want_bw) hard = rcm_batch_bandwidth(b, bw.data());
   if (!hard) {
-    cat.resize((size_t)std::max<int64_t>(b->total_n, 1));
+    if (cat.size() < (size_t)std::max<int64_t>(b->total_n, 1)) cat.resize((size_t)std::max<int64_t>(b->total_n, 1));
     hard = rcm_batch_read_perm(b, nullptr, cat.data());
   }
+  read_ms = ms_since(t);
+  t = clk::now();
   int64_t off = 0;
   for (int32_t k = 0; k < C; ++k) {
     ServerJob& j = *grp[k];
@@ -763,6 +804,14 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     }
     off += j.n;
   }
+  deliver_ms = ms_since(t);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->st.group_host_copy_ms += copy_ms;
+  s->st.group_load_ms += load_ms;
+  s->st.group_order_ms += order_ms;
+  s->st.group_readback_ms += read_ms;
+  s->st.group_deliver_ms += deliver_ms;
+  s->st.group_launches += launches;
 }
 
 void lane_loop(rcm_server* s, int lane) {
@@ -1031,6 +1080,7 @@ int rcm_ctx_set_param(rcm_ctx* c, int param, int64_t v) {
     case RCM_PARAM_STRICT: c->strict = v ? 1 : 0; return RCM_OK;
     case RCM_PARAM_PROFILE: if (v < 0 || v > 2) return RCM_EINVAL; c->profile = v; return RCM_OK;
     case RCM_PARAM_BATCH_CAP: if (v < 0) return RCM_EINVAL; c->batch_cap = v; return RCM_OK;
+    case RCM_PARAM_SPECULATE: if (v < 0) return RCM_EINVAL; c->speculate = v; return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
@@ -1064,6 +1114,7 @@ int rcm_ctx_get_param(rcm_ctx* c, int param, int64_t* v) {
     case RCM_PARAM_STRICT: *v = c->strict; return RCM_OK;
     case RCM_PARAM_PROFILE: *v = c->profile; return RCM_OK;
     case RCM_PARAM_BATCH_CAP: *v = c->batch_cap; return RCM_OK;
+    case RCM_PARAM_SPECULATE: *v = c->speculate; return RCM_OK;
     default: return RCM_EINVAL;
   }
 }
@@ -1122,13 +1173,7 @@ int rcm_ctx_run(rcm_ctx* c, int32_t start, uint32_t flags) {
   c->have_perm = false;
   const int rc = run_engine(c, start, false);
   if (rc) return rc;
-  c->reversed = (flags & RCM_FLAG_REVERSE) != 0;
-  if (c->reversed) {
-    hipLaunchKernelGGL(k_reverse, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->perm, c->n, c->out);
-    RCM_HIP(hipStreamSynchronize(c->stream));
-  }
-  c->have_perm = true;
-  return RCM_OK;
+  return finish_order(c, flags);
 }
 
 // SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:92-141) on the GPU: each sweep is one engine
@@ -1167,6 +1212,72 @@ int rcm_ctx_find_start(rcm_ctx* c, int32_t* start_out, int32_t* iterations_out) 
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->timing.start_iterations = iters;
   *start_out = cur;
+  if (iterations_out) *iterations_out = iters;
+  return RCM_OK;
+}
+
+// Start search and order in one call: the sweeps of SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:
+// 92-141), except that from sweep `speculate` on a sweep is first run as a CM run that records levels. The level
+// of its last queue entry is the eccentricity of that start in any BFS order, so when it equals the previous
+// sweep's the rule stops at this start and the run is the result (one engine pass fewer than rcm_ctx_find_start
+// + rcm_ctx_run); otherwise the plain sweep from the same start follows and the search goes on. Same start,
+// sweep count and order as those two calls.
+int rcm_ctx_order(rcm_ctx* c, int32_t start_node, uint32_t flags, int32_t* start_out, int32_t* iterations_out) {
+  if (!c) return RCM_EINVAL;
+  if (!c->rp || c->n == 0) return RCM_ENOTLOADED;
+  if (start_node >= c->n) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int32_t n = c->n;
+  c->have_perm = false;
+  int32_t cur = start_node, iters = 0, passes = 0;
+  bool ordered = false;
+  if (start_node < 0 && n == 1) {
+    cur = 0;
+    iters = 1;
+  } else if (start_node < 0) {
+    cur = n / 2;
+    uint32_t lastdist = 0;
+    const int32_t w = std::min<int32_t>(32, n - 1);
+    const bool can_spec = c->speculate > 0 && c->variant != kLevelVariant;
+    int32_t nodes[32], dists[32], degs[32];
+    uint64_t rows[32];
+    for (;;) {
+      ++iters;
+      if (can_spec && iters >= c->speculate) {
+        const int rc = run_engine(c, cur, false, true);
+        ++passes;
+        if (rc) return rc;
+        int32_t ecc = -1;
+        RCM_HIP(hipMemcpyAsync(&ecc, c->level + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        RCM_HIP(hipStreamSynchronize(c->stream));
+        if ((uint32_t)ecc == lastdist) {
+          ordered = true;
+          break;
+        }
+      }
+      const int rc = run_engine(c, cur, true);
+      ++passes;
+      if (rc) return rc;
+      RCM_HIP(hipMemcpyAsync(nodes, c->perm + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipMemcpyAsync(dists, c->level + (n - w), (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipMemcpyAsync(rows, c->qinfo + (n - w), (size_t)w * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+      RCM_HIP(hipStreamSynchronize(c->stream));
+      for (int32_t i = 0; i < w; ++i) degs[i] = (int32_t)((uint32_t)(rows[i] >> 32) - (uint32_t)rows[i]);
+      if (peripheral_step(nodes, dists, degs, w, &lastdist, &cur)) break;
+    }
+  }
+  if (!ordered) {
+    const int rc = run_engine(c, cur, false);
+    ++passes;
+    if (rc) return rc;
+  }
+  const int rc = finish_order(c, flags);
+  if (rc) return rc;
+  c->timing.start_iterations = iters;
+  c->timing.order_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->timing.order_passes = passes;
+  if (start_out) *start_out = cur;
   if (iterations_out) *iterations_out = iters;
   return RCM_OK;
 }
@@ -1257,8 +1368,7 @@ int rcm_gpu_batch(int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t*
   if (rc) return rc;
   rc = rcm_ctx_load(c, n, nnz, row_ptr, col_idx);
   int32_t start = start_node;
-  if (!rc && start < 0) rc = rcm_ctx_find_start(c, &start, nullptr);
-  if (!rc) rc = rcm_ctx_run(c, start, flags);
+  if (!rc) rc = rcm_ctx_order(c, start_node, flags, &start, nullptr);
   if (!rc && bandwidth_out && (flags & RCM_FLAG_BANDWIDTH)) rc = rcm_ctx_bandwidth(c, bandwidth_out);
   if (!rc) rc = rcm_ctx_read_perm(c, perm_out);
   if (!rc && start_node_out) *start_node_out = start;
@@ -1291,7 +1401,7 @@ int rcm_batch_create(int device, rcm_batch** out) {
     rcm_batch_destroy(b);
     return RCM_EDEVICE;
   }
-  b->max_resident = b->max_res[0];
+  b->max_resident = b->max_res[b->variant];
   *out = b;
   return RCM_OK;
 }
@@ -1321,6 +1431,7 @@ int rcm_batch_set_param(rcm_batch* b, int param, int64_t v) {
       batch_drop_graphs(b);
       return RCM_OK;
     case RCM_PARAM_GRAPH: b->use_graph = v ? 1 : 0; return RCM_OK;
+    case RCM_PARAM_SPECULATE: if (v < 0) return RCM_EINVAL; b->speculate = v; return RCM_OK;
     case RCM_PARAM_WG_MAX_N:  // reserved: the one-workgroup-per-matrix engine is not part of the library
       return v == 0 ? RCM_OK : RCM_EINVAL;
     case RCM_PARAM_VARIANT:
@@ -1449,6 +1560,35 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   return rc;
 }
 
+namespace {
+// One launch over `list` (plain-BFS sweeps when bfs, else CM runs, which record levels when `levels`) from
+// cur[], then back on the host: every control block, and for each listed matrix the last w = min(32, n-1)
+// queue entries (node, level, row length) at h_tail[i * 96]. Per-matrix status of the listed matrices goes to
+// status[]; a device-level failure is returned.
+int batch_pass(rcm_batch* b, const std::vector<int32_t>& list, const std::vector<int32_t>& cur, bool bfs,
+               bool levels, int32_t* status) {
+  const int32_t C = b->count, A = (int32_t)list.size();
+  RCM_HIP(hipMemcpyAsync(b->d_starts, cur.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->d_active, list.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  std::vector<int32_t> blk;
+  (void)build_blk(b, list.data(), A, blk);
+  RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                         b->stream));
+  int64_t rows = 0;
+  for (int32_t k : list) rows += b->n[k];
+  enqueue_many(b, b->d_active, b->blk_active, blk[A], A, bfs, levels, rows);
+  hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
+  RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->h_g, b->g, sizeof(ManyControl), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+  RCM_HIP(hipEventRecord(b->ev[3], b->stream));
+  const int wr = batch_wait(b, b->ev[3]);
+  if (wr) return wr;
+  (void)collect_status(b, list.data(), A, status, nullptr);
+  return RCM_OK;
+}
+}  // namespace
+
 // SimplePeripheral::findPeripheral (src/simple_peripheral.cpp:92-141) for every matrix: each sweep is one
 // persistent plain-BFS launch over the matrices still iterating; the rule on the last min(32, n-1) queue
 // entries is applied per matrix on the host, as in rcm_ctx_find_start.
@@ -1472,29 +1612,15 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
   int sweeps = 0;
   while (!active.empty()) {
     ++sweeps;
-    const int32_t A = (int32_t)active.size();
-    RCM_HIP(hipMemcpyAsync(b->d_starts, cur.data(), (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-    RCM_HIP(hipMemcpyAsync(b->d_active, active.data(), (size_t)A * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-    std::vector<int32_t> blk;
-    (void)build_blk(b, active.data(), A, blk);
-    RCM_HIP(hipMemcpyAsync(b->blk_active, blk.data(), ((size_t)A + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
-                           b->stream));
-    enqueue_many(b, b->d_active, b->blk_active, blk[A], A, true);
-    hipLaunchKernelGGL(k_gather_tail, dim3((unsigned)A), dim3(64), 0, b->stream, b->jobs, b->d_active, A, b->tail);
-    RCM_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, (size_t)C * sizeof(Control), hipMemcpyDeviceToHost, b->stream));
-    RCM_HIP(hipMemcpyAsync(b->h_g, b->g, sizeof(ManyControl), hipMemcpyDeviceToHost, b->stream));
-    RCM_HIP(hipMemcpyAsync(b->h_tail, b->tail, (size_t)A * 96 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
-    RCM_HIP(hipEventRecord(b->ev[3], b->stream));
-    const int wr = batch_wait(b, b->ev[3]);
-    if (wr) return wr;
-    (void)collect_status(b, active.data(), A, status.data(), nullptr);
+    const int r = batch_pass(b, active, cur, true, false, status.data());
+    if (r) return r;
     std::vector<int32_t> next;
-    for (int32_t i = 0; i < A; ++i) {
+    for (size_t i = 0; i < active.size(); ++i) {
       const int32_t k = active[i];
       ++iters[k];
       if (status[k] != RCM_OK) continue;
       const int32_t w = std::min<int32_t>(32, b->n[k] - 1);
-      const int32_t* nodes = b->h_tail + (size_t)i * 96;
+      const int32_t* nodes = b->h_tail + i * 96;
       if (peripheral_step(nodes, nodes + 32, nodes + 64, w, &lastdist[k], &cur[k])) continue;  // converged
       next.push_back(k);
     }
@@ -1512,6 +1638,102 @@ int rcm_batch_find_start(rcm_batch* b, int32_t* start_out, int32_t* iterations_o
   return rc;
 }
 
+// Start search and order for every matrix, as rcm_batch_find_start + rcm_batch_run with one engine pass fewer
+// per matrix that stops at a speculated sweep (see rcm_ctx_order). Each launch is of one kind: plain sweeps while
+// any matrix needs one, otherwise CM runs (speculated sweeps, and final runs of matrices whose start is known).
+int rcm_batch_order(rcm_batch* b, const int32_t* start_in, uint32_t flags, int32_t* start_out,
+                    int32_t* iterations_out, int32_t* status_out) {
+  if (!b) return RCM_EINVAL;
+  if (b->count == 0) return RCM_ENOTLOADED;
+  const int32_t C = b->count;
+  if (start_in)
+    for (int32_t k = 0; k < C; ++k)
+      if (start_in[k] >= b->n[k]) return RCM_EINVAL;
+  RCM_HIP(hipSetDevice(b->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  b->have_perm = false;
+  enum : int8_t { kPlain, kSpec, kFinal, kDone };
+  std::vector<int32_t> cur(C), iters(C, 0), status(C, RCM_OK);
+  std::vector<uint32_t> lastdist(C, 0);
+  std::vector<int8_t> need(C), redo(C, 0);  // redo: the plain sweep repeats a speculated one (counted already)
+  for (int32_t k = 0; k < C; ++k) {
+    if (start_in && start_in[k] >= 0) {
+      cur[k] = start_in[k];
+      need[k] = kFinal;
+    } else if (b->n[k] == 1) {
+      cur[k] = 0;
+      iters[k] = 1;
+      need[k] = kFinal;
+    } else {
+      cur[k] = b->n[k] / 2;
+      need[k] = kPlain;
+    }
+  }
+  int launches = 0;
+  std::vector<int32_t> plain, cm;
+  for (;;) {
+    plain.clear();
+    cm.clear();
+    for (int32_t k : b->order) {
+      if (need[k] == kPlain) plain.push_back(k);
+      else if (need[k] != kDone) cm.push_back(k);
+    }
+    const bool bfs = !plain.empty();
+    const std::vector<int32_t>& list = bfs ? plain : cm;
+    if (list.empty()) break;
+    ++launches;
+    const int r = batch_pass(b, list, cur, bfs, !bfs, status.data());
+    if (r) return r;
+    for (size_t i = 0; i < list.size(); ++i) {
+      const int32_t k = list[i];
+      const int32_t w = std::min<int32_t>(32, b->n[k] - 1);
+      const int32_t* tail = b->h_tail + i * 96;
+      if (bfs) {
+        if (!redo[k]) ++iters[k];
+        redo[k] = 0;
+        if (status[k] != RCM_OK) {
+          need[k] = kDone;
+        } else if (peripheral_step(tail, tail + 32, tail + 64, w, &lastdist[k], &cur[k])) {
+          need[k] = kFinal;
+        } else {
+          need[k] = (b->speculate > 0 && iters[k] + 1 >= b->speculate) ? kSpec : kPlain;
+        }
+      } else if (need[k] == kFinal || status[k] != RCM_OK) {
+        need[k] = kDone;
+      } else {  // speculated sweep: the last entry's level is the eccentricity of cur
+        ++iters[k];
+        if ((uint32_t)tail[32 + w - 1] == lastdist[k]) {
+          need[k] = kDone;
+        } else {
+          need[k] = kPlain;
+          redo[k] = 1;
+        }
+      }
+    }
+  }
+  const bool rev = (flags & RCM_FLAG_REVERSE) != 0;
+  if (rev) {
+    hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
+                       C, b->blk_order);
+    RCM_HIP(hipStreamSynchronize(b->stream));
+  }
+  int rc = RCM_OK;
+  b->ok_jobs.clear();
+  for (int32_t k : b->order)
+    if (status[k] == RCM_OK) b->ok_jobs.push_back(k);
+  for (int32_t k = 0; k < C; ++k) {
+    if (start_out) start_out[k] = cur[k];
+    if (iterations_out) iterations_out[k] = iters[k];
+    if (status_out) status_out[k] = status[k];
+    if (status[k] && rc == RCM_OK) rc = status[k];
+  }
+  b->reversed = rev;
+  b->have_perm = true;  // per-matrix status tells which orders are complete
+  b->timing.order_passes = launches;
+  b->timing.order_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
 int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* status_out) {
   if (!b || !start) return RCM_EINVAL;
   if (b->count == 0) return RCM_ENOTLOADED;
@@ -1524,7 +1746,7 @@ int rcm_batch_run(rcm_batch* b, const int32_t* start, uint32_t flags, int32_t* s
   RCM_HIP(hipMemcpyAsync(b->d_starts, start, (size_t)C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
   RCM_HIP(hipEventRecord(b->ev[0], b->stream));
   auto enqueue_run = [&]() {
-    enqueue_many(b, b->d_order, b->blk_order, b->nblk_order, C, false);
+    enqueue_many(b, b->d_order, b->blk_order, b->nblk_order, C, false, false, b->total_n);
     if (rev)
       hipLaunchKernelGGL(k_reverse_many, dim3((unsigned)b->nblk_order), dim3(256), 0, b->stream, b->jobs, b->d_order,
                          C, b->blk_order);
@@ -1653,32 +1875,19 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
   rcm_batch_set_param(b, RCM_PARAM_GRAPH, 0);  // one run: nothing to replay
   std::vector<int32_t> st((size_t)count, 0), status((size_t)count, RCM_OK);
   rc = rcm_batch_load(b, count, n, nnz, rp_off, ci_off, row_ptr_cat, col_idx_cat, status.data());
-  bool need_search = false;
-  for (int32_t k = 0; k < count; ++k) {
+  for (int32_t k = 0; k < count; ++k)
     if (start && start[k] >= n[k]) {
       status[k] = RCM_EINVAL;
       if (!rc) rc = RCM_EINVAL;
     }
-    if (!start || start[k] < 0) need_search = true;
-  }
   // a matrix that fails (e.g. disconnected) keeps its status; the others are still reordered
-  std::vector<int32_t> st_search((size_t)count, RCM_OK), st_run((size_t)count, RCM_OK);
   int hard = rc;  // launch-level failure (malformed input, device error, time-out)
-  if (!hard && need_search) {
-    const int r = rcm_batch_find_start(b, st.data(), nullptr, st_search.data());
-    if (r == RCM_EDEVICE || r == RCM_ETIMEOUT) hard = r;
-  }
   int worst = hard;
-  if (!hard) {
-    if (start)
-      for (int32_t k = 0; k < count; ++k)
-        if (start[k] >= 0) st[k] = start[k];
-    const int r = rcm_batch_run(b, st.data(), flags, st_run.data());
+  if (!hard) {  // start search where no start is given, and order, in one call
+    const int r = rcm_batch_order(b, start, flags, st.data(), nullptr, status.data());
     if (r == RCM_EDEVICE || r == RCM_ETIMEOUT) hard = r;
-    for (int32_t k = 0; k < count; ++k) {
-      status[k] = (start && start[k] >= 0) ? st_run[k] : (st_search[k] ? st_search[k] : st_run[k]);
+    for (int32_t k = 0; k < count; ++k)
       if (status[k] && worst == RCM_OK) worst = status[k];
-    }
   }
   if (!hard && bandwidth_out && (flags & RCM_FLAG_BANDWIDTH)) hard = rcm_batch_bandwidth(b, bandwidth_out);
   if (!hard) hard = rcm_batch_read_perm(b, po_off, perm_cat);
@@ -1721,7 +1930,11 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
     rc = rcm_batch_create(device, &b);
     if (rc) break;
     s->batch.push_back(b);
-    const int w = c.workers > 0 ? c.workers : std::max(1, b->max_resident / s->lanes);
+    rcm_batch_set_param(b, RCM_PARAM_VARIANT, 0);  // lanes were measured with variant 0 (256 threads)
+    // twice the lane's share by default: idle workgroups leave a launch (k_reorder_many), so a lane whose group
+    // needs fewer gives its slots back; measured on the 400-request stream, 4 lanes: capacity 4.0x one-at-a-time
+    // with 128 workgroups per lane, 4.4x with 256 or 384, 3.9x with 512
+    const int w = c.workers > 0 ? c.workers : std::min(b->max_resident, std::max(1, 2 * b->max_resident / s->lanes));
     rcm_batch_set_param(b, RCM_PARAM_WORKERS, w);
     rcm_batch_set_param(b, RCM_PARAM_TIMEOUT_MS, c.job_timeout_ms);
     rcm_batch_set_param(b, RCM_PARAM_GRAPH, 0);  // every group is a different set
@@ -1738,6 +1951,9 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
   s->st.workers = wtotal;
   s->st.arena_bytes = (int64_t)s->lane_bytes * s->lanes;
   s->st.kernel_running = 0;
+  s->stage_rp.resize((size_t)s->lanes);
+  s->stage_ci.resize((size_t)s->lanes);
+  s->stage_perm.resize((size_t)s->lanes);
   for (int l = 0; l < s->lanes; ++l) s->threads.emplace_back(lane_loop, s, l);
   *out = s;
   return RCM_OK;

@@ -9,20 +9,20 @@ using namespace rcm;
 
 namespace {
 // the library's engine variants: (threads, adjacency slots, parents, digit bits)
-template <class G, int D, bool BFS>
+template <class G, int D, bool BFS, bool LEV>
 void run_reorder(unsigned workers, RunArgs a) {
   emu::emu_lds_bytes = sizeof(Smem<G, D>);
-  emu::launch(workers, G::T, true, k_reorder<G, D, false, BFS, false>, a);
+  emu::launch(workers, G::T, true, k_reorder<G, D, false, BFS, false, LEV>, a);
   emu::emu_lds_bytes = 256 * 1024;
 }
-template <bool BFS>
+template <bool BFS, bool LEV = false>
 bool run_variant(int variant, unsigned workers, RunArgs a) {
   switch (variant) {
-    case 0: run_reorder<Geo<256, 2048, 256>, 4, BFS>(workers, a); return true;
-    case 1: run_reorder<Geo<256, 4096, 256>, 4, BFS>(workers, a); return true;
-    case 2: run_reorder<Geo<128, 1024, 128>, 4, BFS>(workers, a); return true;
-    case 3: run_reorder<Geo<256, 2048, 256>, 6, BFS>(workers, a); return true;
-    case 4: run_reorder<Geo<512, 4096, 512>, 4, BFS>(workers, a); return true;
+    case 0: run_reorder<Geo<256, 2048, 256>, 4, BFS, LEV>(workers, a); return true;
+    case 1: run_reorder<Geo<256, 4096, 256>, 4, BFS, LEV>(workers, a); return true;
+    case 2: run_reorder<Geo<128, 1024, 128>, 4, BFS, LEV>(workers, a); return true;
+    case 3: run_reorder<Geo<256, 2048, 256>, 6, BFS, LEV>(workers, a); return true;
+    case 4: run_reorder<Geo<512, 4096, 512>, 4, BFS, LEV>(workers, a); return true;
     default: return false;
   }
 }
@@ -58,7 +58,10 @@ int emu_run(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, int32_
   emu::launch(1, 64, false, k_init, a, start, n + 1);
   // `slots` selects the variant: 2048 -> 0, 4096 -> 1, 1024 -> 2; 2049 -> 3 (6-bit digits), 4097 -> 4
   const int variant = slots == 2048 ? 0 : slots == 4096 ? 1 : slots == 1024 ? 2 : slots == 2049 ? 3 : slots == 4097 ? 4 : -1;
-  if (!(bfs ? run_variant<true>(variant, workers, a) : run_variant<false>(variant, workers, a))) return -1;
+  // bfs: 0 = CM, 1 = plain BFS with levels, 2 = CM with levels
+  if (!(bfs == 1 ? run_variant<true>(variant, workers, a)
+                  : bfs == 2 ? run_variant<false, true>(variant, workers, a) : run_variant<false>(variant, workers, a)))
+    return -1;
   std::memcpy(perm_out, perm.data(), sizeof(int32_t) * n);
   if (level_out) std::memcpy(level_out, level.data(), sizeof(int32_t) * n);
   info[0] = (int32_t)ctl.error;

@@ -62,12 +62,17 @@ int emu_run_many(int32_t count, const int32_t* n, const int64_t* nnz, const int6
   m.budget_ticks = (uint64_t)(timeout_s * 1e8);
   m.sort_mode = 0;
   m.seg_threshold = 24;
+  m.share_workers = (uint32_t)workers;  // the size-proportional helper floor, as the library's launches use it
+  m.share_rows = (uint64_t)tot;
   std::vector<int32_t> blk(count + 1);
   for (int k = 0; k <= count; ++k) blk[k] = 2 * k;  // two workgroups per job
   emu::launch((unsigned)count * 2, 64, false, k_init_many, m, (const int32_t*)blk.data());
   emu::emu_lds_bytes = sizeof(ManySmem<GeoLib, 4>);
-  if (bfs)
+  // bfs: 0 = CM, 1 = plain BFS with levels, 2 = CM with levels
+  if (bfs == 1)
     emu::launch((unsigned)workers, GeoLib::T, true, k_reorder_many<GeoLib, 4, false, true>, m);
+  else if (bfs == 2)
+    emu::launch((unsigned)workers, GeoLib::T, true, k_reorder_many<GeoLib, 4, false, false, true>, m);
   else
     emu::launch((unsigned)workers, GeoLib::T, true, k_reorder_many<GeoLib, 4, false, false>, m);
   emu::emu_lds_bytes = 256 * 1024;

@@ -32,8 +32,9 @@ def lib(engine: str = "block"):
 
 
 def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=20.0, engine="block",
-        claim_filter=0):
-    """Returns (perm, level, info) with info = [error, placed, batches, diag...]."""
+        claim_filter=0, levels=False):
+    """Returns (perm, level, info) with info = [error, placed, batches, diag...]. levels: a CM run that also
+    records BFS levels (block engine)."""
     if slots is None:
         slots = 2048 if engine == "block" else 1024
     rp = np.ascontiguousarray(rp, np.int32)
@@ -43,7 +44,7 @@ def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=
     lev = np.full(n, -1, np.int32)
     info = np.zeros(12, np.int32)
     rc = lib(engine).emu_run(n, ci.shape[0], rp.ctypes.data_as(I32P), ci.ctypes.data_as(I32P), int(start), workers,
-                             slots, sort_mode, 1 if bfs else 0, perm.ctypes.data_as(I32P),
+                             slots, sort_mode, 1 if bfs else (2 if levels else 0), perm.ctypes.data_as(I32P),
                              lev.ctypes.data_as(I32P), info.ctypes.data_as(I32P), timeout_s, claim_filter)
     assert rc == 0, "emulator rejected the arguments"
     return perm, lev, info
@@ -60,8 +61,9 @@ def _many_lib():
     return L
 
 
-def run_many(mats, starts, workers=2, bfs=False, timeout_s=30.0):
-    """mats: list of (rp, ci). Returns (list of perms, list of levels, info[count, 4], jobs_done)."""
+def run_many(mats, starts, workers=2, bfs=False, timeout_s=30.0, levels=False):
+    """mats: list of (rp, ci). Returns (list of perms, list of levels, info[count, 4], jobs_done). levels: CM
+    runs that also record BFS levels."""
     I64P = C.POINTER(C.c_int64)
     n = np.array([m[0].shape[0] - 1 for m in mats], np.int32)
     nnz = np.array([m[1].shape[0] for m in mats], np.int64)
@@ -77,7 +79,8 @@ def run_many(mats, starts, workers=2, bfs=False, timeout_s=30.0):
     done = _many_lib().emu_run_many(len(mats), n.ctypes.data_as(I32P), nnz.ctypes.data_as(I64P),
                                     rp_off.ctypes.data_as(I64P), ci_off.ctypes.data_as(I64P),
                                     rp_cat.ctypes.data_as(I32P), ci_cat.ctypes.data_as(I32P), st.ctypes.data_as(I32P),
-                                    workers, 1 if bfs else 0, perm.ctypes.data_as(I32P), lev.ctypes.data_as(I32P),
+                                    workers, 1 if bfs else (2 if levels else 0), perm.ctypes.data_as(I32P),
+                                    lev.ctypes.data_as(I32P),
                                     info.ctypes.data_as(I32P), timeout_s)
     offs = np.concatenate([[0], np.cumsum(n)])
     perms = [perm[offs[k]:offs[k + 1]] for k in range(len(mats))]

@@ -106,6 +106,47 @@ def test_many_matrices_bfs_levels():
         assert np.array_equal(levs[k], level), k
 
 
+def _dist_by_node(rp, ci, start):
+    """BFS distance of every node from start, from the oracle's plain BFS (order, level per position)."""
+    order, level = O.bfs(rp, ci, start)
+    dist = np.full(rp.shape[0] - 1, -1, np.int64)
+    dist[order] = level
+    return dist
+
+
+@pytest.mark.parametrize("name", ["stencil24x20", "duplicates300", "grid9x7"])
+def test_cm_run_records_levels(name):
+    """A CM run with a level buffer (the start search's speculative sweep) places the CM order unchanged and
+    records each position's BFS distance from the start; its last entry is the start's eccentricity."""
+    rp, ci = CASES[name]
+    for start in _starts(rp, ci):
+        perm, lev, info = emu_lib.run(rp, ci, start, workers=3, levels=True)
+        assert info[0] == 0
+        assert np.array_equal(perm, O.cm(rp, ci, start)), (name, start)
+        dist = _dist_by_node(rp, ci, start)
+        assert np.array_equal(lev, dist[perm]), (name, start)
+        assert lev[-1] == dist.max()
+
+
+def test_cm_run_records_levels_long_rows():
+    rp, ci = LONG["twohubs2600"]()
+    for start in (5, int(np.argmax(np.diff(rp)))):
+        perm, lev, info = emu_lib.run(rp, ci, start, workers=2, levels=True, timeout_s=60)
+        assert info[0] == 0
+        assert np.array_equal(perm, O.cm(rp, ci, start))
+        assert np.array_equal(lev, _dist_by_node(rp, ci, start)[perm])
+
+
+def test_many_matrices_cm_levels():
+    mats = _many_set() + [LONG["star3000"]()]
+    starts = [0 for _ in mats]
+    perms, levs, info, done = emu_lib.run_many(mats, starts, workers=4, levels=True, timeout_s=60)
+    assert done == len(mats)
+    for k, (rp, ci) in enumerate(mats):
+        assert np.array_equal(perms[k], O.cm(rp, ci, 0)), k
+        assert np.array_equal(levs[k], _dist_by_node(rp, ci, 0)[perms[k]]), k
+
+
 def test_many_matrices_disconnected_member():
     rp, ci = G.from_edges(10, [(0, 1), (1, 2), (5, 6)])
     mats = [G.path(9), (rp, ci), G.star(12)]
