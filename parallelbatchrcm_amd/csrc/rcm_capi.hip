@@ -43,6 +43,9 @@ constexpr int64_t kDefaultSpeculate = 3;
 // Serving lanes give each request more helpers: latency, not throughput, is the aim there. Measured on MI355X
 // (Poisson stream, 95% mixed sizes and 5% R-MAT): 2048 against 8192 halves p50 at 200 and 400 requests/s.
 constexpr int64_t kServerHelperNodes = 2048;
+// Per serving lane, pinned host staging for a group's concatenated CSRs (uploaded by DMA, without the runtime's
+// bounce copy through its own pinned buffers) and for its orders; larger groups stage in pageable memory.
+constexpr size_t kServePinnedWords = (size_t)64 << 20;  // 256 MiB per lane
 // Serving groups requests of at least this many rows only with each other.
 constexpr int32_t kServeLargeN = 1 << 18;
 
@@ -719,8 +722,9 @@ struct rcm_server {
   int64_t inflight_bytes = 0;
   rcm_server_stats st{};
   // per lane, grow-only host staging for a group's concatenated CSRs and orders: reused, so a group does not
-  // pay for fresh zero-filled pages
+  // pay for fresh zero-filled pages; and the lane's pinned staging (kServePinnedWords, null if not allocated)
   std::vector<std::vector<int32_t>> stage_rp, stage_ci, stage_perm;
+  std::vector<int32_t*> pinned;
 };
 
 namespace {
@@ -746,18 +750,27 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     cin += nnz[k];
     want_bw = want_bw || (grp[k]->flags & RCM_FLAG_BANDWIDTH);
   }
-  std::vector<int32_t>& rp_cat = s->stage_rp[(size_t)lane];
-  std::vector<int32_t>& ci_cat = s->stage_ci[(size_t)lane];
-  if (rp_cat.size() < (size_t)rpn) rp_cat.resize((size_t)rpn);
-  if (ci_cat.size() < (size_t)std::max<int64_t>(cin, 1)) ci_cat.resize((size_t)std::max<int64_t>(cin, 1));
+  int32_t* pin = s->pinned[(size_t)lane];
+  const bool use_pin = pin && (size_t)(rpn + std::max<int64_t>(cin, 1)) <= kServePinnedWords;
+  int32_t *rp_cat, *ci_cat;
+  if (use_pin) {
+    rp_cat = pin;
+    ci_cat = pin + rpn;
+  } else {
+    std::vector<int32_t>& rv = s->stage_rp[(size_t)lane];
+    std::vector<int32_t>& cv = s->stage_ci[(size_t)lane];
+    if (rv.size() < (size_t)rpn) rv.resize((size_t)rpn);
+    if (cv.size() < (size_t)std::max<int64_t>(cin, 1)) cv.resize((size_t)std::max<int64_t>(cin, 1));
+    rp_cat = rv.data();
+    ci_cat = cv.data();
+  }
   for (int32_t k = 0; k < C; ++k) {
-    std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat.begin() + rp_off[k]);
-    std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat.begin() + ci_off[k]);
+    std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat + rp_off[k]);
+    std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat + ci_off[k]);
   }
   copy_ms = ms_since(t);
   t = clk::now();
-  int hard = rcm_batch_load(b, C, n.data(), nnz.data(), rp_off.data(), ci_off.data(), rp_cat.data(), ci_cat.data(),
-                            st.data());
+  int hard = rcm_batch_load(b, C, n.data(), nnz.data(), rp_off.data(), ci_off.data(), rp_cat, ci_cat, st.data());
   load_ms = ms_since(t);
   std::vector<int32_t> iters((size_t)C, 0);
   if (hard == RCM_EINVAL) {  // some request is malformed: report it, run the rest without it
@@ -783,11 +796,19 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
   order_ms = ms_since(t);
   t = clk::now();
   std::vector<int64_t> bw((size_t)C, -1);
-  std::vector<int32_t>& cat = s->stage_perm[(size_t)lane];
+  // the orders: into the pinned staging when they fit (the load above has completed, so it is free again)
+  int32_t* cat = nullptr;
   if (!hard && want_bw) hard = rcm_batch_bandwidth(b, bw.data());
   if (!hard) {
-    if (cat.size() < (size_t)std::max<int64_t>(b->total_n, 1)) cat.resize((size_t)std::max<int64_t>(b->total_n, 1));
-    hard = rcm_batch_read_perm(b, nullptr, cat.data());
+    const size_t words = (size_t)std::max<int64_t>(b->total_n, 1);
+    if (pin && words <= kServePinnedWords) {
+      cat = pin;
+    } else {
+      std::vector<int32_t>& pv = s->stage_perm[(size_t)lane];
+      if (pv.size() < words) pv.resize(words);
+      cat = pv.data();
+    }
+    hard = rcm_batch_read_perm(b, nullptr, cat);
   }
   read_ms = ms_since(t);
   t = clk::now();
@@ -798,7 +819,7 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     j.sweeps = iters[k];
     j.status = hard ? hard : (st[k] ? st[k] : st2[k]);
     if (j.status == RCM_OK) {
-      j.perm.assign(cat.begin() + off, cat.begin() + off + j.n);
+      j.perm.assign(cat + off, cat + off + j.n);
       if (j.flags & RCM_FLAG_REVERSE) std::reverse(j.perm.begin(), j.perm.end());
       if (j.flags & RCM_FLAG_BANDWIDTH) j.bw = bw[k];
     }
@@ -1954,6 +1975,13 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
   s->stage_rp.resize((size_t)s->lanes);
   s->stage_ci.resize((size_t)s->lanes);
   s->stage_perm.resize((size_t)s->lanes);
+  s->pinned.assign((size_t)s->lanes, nullptr);
+  for (int l = 0; l < s->lanes; ++l)  // optional: a lane without it stages in pageable memory
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned[(size_t)l]), kServePinnedWords * sizeof(int32_t),
+                      hipHostMallocDefault) != hipSuccess) {
+      s->pinned[(size_t)l] = nullptr;
+      (void)hipGetLastError();  // clear the failed call's error
+    }
   for (int l = 0; l < s->lanes; ++l) s->threads.emplace_back(lane_loop, s, l);
   *out = s;
   return RCM_OK;
@@ -2045,6 +2073,8 @@ int rcm_server_destroy(rcm_server* s) {
     if (t.joinable()) t.join();  // admitted groups finish first
   s->cv_done.notify_all();
   for (rcm_batch* b : s->batch) rcm_batch_destroy(b);
+  for (int32_t* p : s->pinned)
+    if (p) (void)hipHostFree(p);
   delete s;
   return RCM_OK;
 }
