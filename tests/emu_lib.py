@@ -61,7 +61,7 @@ def _many_lib():
     return L
 
 
-def run_many(mats, starts, workers=2, bfs=False, timeout_s=30.0, levels=False):
+def run_many(mats, starts, workers=2, bfs=False, timeout_s=60.0, levels=False):
     """mats: list of (rp, ci). Returns (list of perms, list of levels, info[count, 4], jobs_done). levels: CM
     runs that also record BFS levels."""
     I64P = C.POINTER(C.c_int64)
