@@ -197,6 +197,27 @@ def test_batch_autotune_keeps_results():
         assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, int(starts[k]))), k
 
 
+def test_batch_autotune_with_disconnected_member():
+    """A member that fails (disconnected) does not void the timings: every candidate is timed over the whole
+    set, the best time is positive and no worse than the default's, and the connected members stay exact."""
+    rp, ci = G.from_edges(12, [(0, 1), (1, 2), (2, 3), (6, 7), (7, 8)])
+    mats = [P.CSR(rp, ci)] + [P.workloads.config(3, scale=4000 + 500 * k, seed=40 + k) for k in range(4)]
+    bs = P.BatchSet.from_matrices(mats)
+    _say("batch autotune with a disconnected member")
+    with P.RcmBatch(timeout_ms=30000) as b:
+        b.load(bs)
+        starts = [0] + [O.peripheral(m.row_offsets, m.col_ids)[1] for m in mats[1:]]
+        r = b.autotune(starts)
+        assert r.candidates >= 10 and 0 < r.best_ms <= r.default_ms
+        assert r.helper_nodes in (8192, 32768, 131072)
+        st = b.run(starts)
+        assert st[0] == P.RCM_ENOTCONNECTED and (st[1:] == 0).all()
+        perms = b.perms()
+    for k in range(1, len(mats)):
+        m = mats[k]
+        assert np.array_equal(perms[k], O.cm(m.row_offsets, m.col_ids, int(starts[k]))), k
+
+
 @pytest.mark.skip(reason="the one-workgroup-per-matrix engine (tests/emu/engine_wg.hpp) faulted the GPU on its "
                          "first run with no cause shown; it is not built into the library (wg_max_n > 0 is "
                          "rejected) and is tested only in the CPU emulator")
