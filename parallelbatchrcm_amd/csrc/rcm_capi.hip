@@ -764,9 +764,32 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     rp_cat = rv.data();
     ci_cat = cv.data();
   }
-  for (int32_t k = 0; k < C; ++k) {
-    std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat + rp_off[k]);
-    std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat + ci_off[k]);
+  // the concatenation: large groups are split over kCopyThreads host threads by bytes (the lanes' threads and
+  // these stay within the 16 cores a GPU's host share has)
+  auto copy_range = [&](int32_t k0, int32_t k1) {
+    for (int32_t k = k0; k < k1; ++k) {
+      std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat + rp_off[k]);
+      std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat + ci_off[k]);
+    }
+  };
+  constexpr int kCopyThreads = 4;
+  const int64_t words = rpn + cin;
+  if (words >= ((int64_t)1 << 18) && C > 1) {
+    std::vector<std::thread> th;
+    int32_t k0 = 0;
+    for (int p = 0; p < kCopyThreads && k0 < C; ++p) {
+      const int64_t goal = words * (p + 1) / kCopyThreads;  // split by the running word count
+      int32_t k1 = k0;
+      while (k1 < C && (p == kCopyThreads - 1 || rp_off[k1] + ci_off[k1] < goal)) ++k1;
+      if (k1 == k0) k1 = k0 + 1;
+      if (p == kCopyThreads - 1) k1 = C;
+      th.emplace_back(copy_range, k0, k1);
+      k0 = k1;
+    }
+    if (k0 < C) copy_range(k0, C);
+    for (auto& x : th) x.join();
+  } else {
+    copy_range(0, C);
   }
   copy_ms = ms_since(t);
   t = clk::now();
