@@ -598,11 +598,13 @@ __device__ __forceinline__ void cc_link(uint32_t* parent, uint32_t u, uint32_t v
   }
 }
 
-// Every edge (u, v), v < u, of each row (sorted columns: each undirected edge once, from its larger end).
+// Every edge (u, v), v < u, of each row (sorted columns: each undirected edge once, from its larger end), except
+// the row's first entry when it is below u: k_cc_init already made it u's parent.
 __global__ __launch_bounds__(kT) void k_cc_hook(const int32_t* rp, const int32_t* ci, int32_t n, uint32_t* parent) {
   const int64_t u = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (u >= n) return;
-  for (int32_t k = rp[u]; k < rp[u + 1]; ++k) {
+  const int32_t k0 = rp[u], k1 = rp[u + 1];
+  for (int32_t k = (k0 < k1 && ci[k0] < (int32_t)u) ? k0 + 1 : k0; k < k1; ++k) {
     const uint32_t v = (uint32_t)ci[k];
     if (v >= (uint32_t)u) break;
     cc_link(parent, (uint32_t)u, v);
