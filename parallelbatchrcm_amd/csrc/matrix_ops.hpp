@@ -502,24 +502,92 @@ __global__ __launch_bounds__(kT) void k_empty_rows(const int32_t* rp, int32_t n,
   if (r < n && rp[r] == rp[r + 1]) *flag = 1u;
 }
 
-// COO in the insertion order of the symmetrisation: each entry, then its transpose when its pair is odd.
-// opos: scanned odd flags (nnz + 1 entries).
+// The symmetrised CSR without a second sort of every entry. The reference inserts each entry and, when its pair is
+// odd, its transpose right after it, and converts that COO stably, so a row of the result is ordered by (column,
+// insertion position). Entry q of the row-sorted CSR has position P(q) = q + opos[q] (opos: scanned odd flags,
+// nnz + 1 entries) and its transpose P(q) + 1. Row r of the result holds row r's entries, already in (column,
+// position) order, and the transposes landing in row r, in the same order once sorted stably by that row (their
+// sources come in row order). An entry's place is its index in its group plus the number of entries of the other
+// group before it, by binary search: no per-row sort, and hub rows stay parallel.
+
+// The transposes (c, r) of the odd entries (r, c): key c << 32 | r, payload the source entry.
+__global__ __launch_bounds__(kT) void k_sym_transposes(const uint64_t* keys, const uint32_t* opos, int64_t nnz,
+                                                      uint64_t* tkeys, uint32_t* tsrc) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= nnz || opos[q + 1] == opos[q]) return;
+  const uint64_t key = keys[q];
+  const uint32_t t = opos[q];
+  tkeys[t] = (key << 32) | (key >> 32);
+  tsrc[t] = (uint32_t)q;
+}
+
+// For the row-sorted transposes: their (column, position) as one comparable word.
+__global__ __launch_bounds__(kT) void k_sym_tkey(const uint64_t* tkeys, const uint32_t* tsrc, const uint32_t* opos,
+                                                int64_t nt, uint64_t* tcp) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= nt) return;
+  const uint32_t src = tsrc[t];
+  tcp[t] = ((uint64_t)(uint32_t)tkeys[t] << 32) | (uint64_t)(src + opos[src] + 1u);
+}
+
+// Row lengths of the result (s rows): row r's entries (r < rows) and its transposes; entry s is 0 for the scan.
+__global__ __launch_bounds__(kT) void k_sym_lengths(const int32_t* rp, int32_t rows, const int32_t* tstart, int32_t s,
+                                                   uint32_t* len) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r > s) return;
+  len[r] = r == s ? 0u
+                  : (uint32_t)((r < rows ? rp[r + 1] - rp[r] : 0) + (tstart[r + 1] - tstart[r]));
+}
+
+// Entry q of the row-sorted CSR into the result (rpo: its row offsets).
 template <class V>
-__global__ __launch_bounds__(kT) void k_sym_expand(const uint64_t* keys, const V* vals, const uint32_t* opos,
-                                                  int64_t nnz, int32_t* r_out, int32_t* c_out, V* v_out) {
+__global__ __launch_bounds__(kT) void k_sym_place_entries(const uint64_t* keys, const uint32_t* opos, int64_t nnz,
+                                                         const int32_t* rp, const uint64_t* tcp,
+                                                         const int32_t* tstart, const int32_t* rpo, const V* vals,
+                                                         int32_t* ci_out, V* v_out) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (q >= nnz) return;
   const uint64_t key = keys[q];
-  const int32_t r = (int32_t)(key >> 32), c = (int32_t)(uint32_t)key;
-  const int64_t p = q + (int64_t)opos[q];
-  r_out[p] = r;
-  c_out[p] = c;
-  if (vals) v_out[p] = vals[q];
-  if (opos[q + 1] != opos[q]) {
-    r_out[p + 1] = c;
-    c_out[p + 1] = r;
-    if (vals) v_out[p + 1] = vals[q];
+  const int32_t r = (int32_t)(key >> 32);
+  const uint64_t me = (key << 32) | (uint64_t)((uint32_t)q + opos[q]);
+  int64_t lo = tstart[r], hi = tstart[r + 1];  // transposes of row r before this entry
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (tcp[mid] < me) lo = mid + 1;
+    else hi = mid;
   }
+  const int64_t pos = (int64_t)rpo[r] + (q - rp[r]) + (lo - tstart[r]);
+  ci_out[pos] = (int32_t)(uint32_t)key;
+  if (vals) v_out[pos] = vals[q];
+}
+
+// Transpose t (row-sorted) into the result.
+template <class V>
+__global__ __launch_bounds__(kT) void k_sym_place_transposes(const uint64_t* tkeys, const uint64_t* tcp,
+                                                            const uint32_t* tsrc, int64_t nt, const int32_t* tstart,
+                                                            const int32_t* rp, const int32_t* ci, const uint32_t* opos,
+                                                            int32_t rows, const int32_t* rpo, const V* vals,
+                                                            int32_t* ci_out, V* v_out) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t r = (int32_t)(tkeys[t] >> 32);
+  const uint64_t me = tcp[t];
+  int64_t lo = 0, hi = 0;  // entries of row r before this transpose
+  if (r < rows) {
+    const int64_t b = rp[r];
+    lo = b;
+    hi = rp[r + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const uint64_t w = ((uint64_t)(uint32_t)ci[mid] << 32) | (uint64_t)((uint32_t)mid + opos[mid]);
+      if (w < me) lo = mid + 1;
+      else hi = mid;
+    }
+    lo -= b;
+  }
+  const int64_t pos = (int64_t)rpo[r] + (t - tstart[r]) + lo;
+  ci_out[pos] = (int32_t)(me >> 32);
+  if (vals) v_out[pos] = vals[tsrc[t]];
 }
 
 // flag[0] = 1: an empty row; flag[1] = 1: an entry (r, c) without (c, r) (columns are sorted)
@@ -954,18 +1022,38 @@ int canonicalise(X& x, int32_t rows, int32_t cols, int64_t nnz, const int32_t* r
     const uint32_t extra = x.read_u32(tot + 1);
     if (x.failed()) return kOpsDevice;
     if (extra > 0) {
-      const int64_t nnz2 = nnz + (int64_t)extra;
+      const int64_t nt = (int64_t)extra;
+      const int64_t nnz2 = nnz + nt;
       if (nnz2 > INT32_MAX) return kOpsInvalid;
-      int32_t* r2 = x.template alloc<int32_t>((size_t)nnz2);
-      int32_t* c2 = x.template alloc<int32_t>((size_t)nnz2);
-      V* v2 = vals ? x.template alloc<V>((size_t)nnz2) : nullptr;
-      if (x.failed()) return kOpsDevice;
-      x.run(k_sym_expand<V>, groups(nnz, kT), (const uint64_t*)skeys, (const V*)a.vals, (const uint32_t*)odd, nnz,
-            r2, c2, v2);
       const int32_t s = rows > cols ? rows : cols;
+      uint64_t* tk = x.template alloc<uint64_t>((size_t)nt);
+      uint64_t* tk2 = x.template alloc<uint64_t>((size_t)nt);
+      uint32_t* ts = x.template alloc<uint32_t>((size_t)nt);
+      uint32_t* ts2 = x.template alloc<uint32_t>((size_t)nt);
+      uint64_t* tcp = x.template alloc<uint64_t>((size_t)nt);
+      int32_t* tstart = x.template alloc<int32_t>((size_t)s + 1);
       Csr<V> b;
-      rc = coo_to_csr(x, s, s, nnz2, r2, c2, (const V*)v2, b, (uint64_t**)nullptr);
-      if (rc) return rc;
+      b.rows = s;
+      b.cols = s;
+      b.nnz = nnz2;
+      b.rp = x.template keep<int32_t>((size_t)s + 1);
+      b.ci = x.template keep<int32_t>((size_t)nnz2);
+      b.vals = vals ? x.template keep<V>((size_t)nnz2) : nullptr;
+      if (x.failed()) return kOpsDevice;
+      x.run(k_sym_transposes, groups(nnz, kT), (const uint64_t*)skeys, (const uint32_t*)odd, nnz, tk, ts);
+      radix_sort(x, tk, ts, tk2, ts2, nt, 0, bits_of((uint64_t)(s - 1)));  // by the transpose's row, stably
+      x.run(k_row_starts, groups(nt, kT), (const uint64_t*)tk, nt, s, tstart);
+      x.run(k_sym_tkey, groups(nt, kT), (const uint64_t*)tk, (const uint32_t*)ts, (const uint32_t*)odd, nt, tcp);
+      uint32_t* len = reinterpret_cast<uint32_t*>(b.rp);
+      x.run(k_sym_lengths, groups((int64_t)s + 1, kT), (const int32_t*)a.rp, rows, (const int32_t*)tstart, s, len);
+      scan_u32(x, len, len, (int64_t)s + 1, (uint32_t*)nullptr);
+      x.run(k_sym_place_entries<V>, groups(nnz, kT), (const uint64_t*)skeys, (const uint32_t*)odd, nnz,
+            (const int32_t*)a.rp, (const uint64_t*)tcp, (const int32_t*)tstart, (const int32_t*)b.rp,
+            (const V*)a.vals, b.ci, b.vals);
+      x.run(k_sym_place_transposes<V>, groups(nt, kT), (const uint64_t*)tk, (const uint64_t*)tcp, (const uint32_t*)ts,
+            nt, (const int32_t*)tstart, (const int32_t*)a.rp, (const int32_t*)a.ci, (const uint32_t*)odd, rows,
+            (const int32_t*)b.rp, (const V*)a.vals, b.ci, b.vals);
+      if (x.failed()) return kOpsDevice;
       a = b;
     }
   }

@@ -164,6 +164,32 @@ def test_canonicalize_closed_forms():
     assert rc == -1
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_canonicalize_nonsquare_symmetrised(seed):
+    """Fewer rows than columns: the symmetrisation places the transposes of entries in the extra columns into
+    rows beyond the input's, and the result is max(rows, cols) square (include/Graph.h:77-157). Checked against the
+    host convert + prepareRCM (themselves equal to the reference's code in tests/test_formats.py)."""
+    rng = np.random.default_rng(90 + seed)
+    cols = int(rng.integers(80, 300))
+    rows = cols - int(rng.integers(1, 20))
+    k = 4 * cols
+    r = rng.integers(0, rows, k).astype(np.int32)
+    c = rng.integers(0, cols, k).astype(np.int32)
+    r = np.concatenate([r, np.arange(rows, dtype=np.int32)])  # a diagonal, and every extra column used
+    c = np.concatenate([c, np.arange(rows, dtype=np.int32)])
+    r = np.concatenate([r, rng.integers(0, rows, cols - rows).astype(np.int32)])
+    c = np.concatenate([c, np.arange(rows, cols, dtype=np.int32)])
+    v = rng.integers(-99, 99, r.size).astype(np.float64)
+    rc, res = E.canonicalize(rows, cols, r, c, v, True)
+    host, hlinks = P.prepare_rcm(P.convert_coo(rows, cols, r, c, v), True)
+    assert rc == 0, rc
+    nn, rp, ci, vv, links = res
+    assert nn == cols and links == hlinks
+    np.testing.assert_array_equal(rp, host.row_offsets)
+    np.testing.assert_array_equal(ci, host.col_ids)
+    np.testing.assert_array_equal(vv, host.data)
+
+
 LEVEL_CASES = {
     "path17": G.path(17),
     "star40": G.star(40),
