@@ -212,7 +212,7 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int,
     print(f"[bench] serving baseline: one at a time {out['one_at_a_time_per_s']:.1f} requests/s (warm context), "
           f"one-shot calls {out['one_shot_call_per_s']:.1f}/s", file=sys.stderr, flush=True)
 
-    def serve(gaps):
+    def serve(gaps, borrow=True):
         lat, dev, que, res = [], [], [], [None] * len(reqs)
         with P.RcmServer(lanes=lanes, workers=workers) as srv:
             ids = [None] * len(reqs)
@@ -239,15 +239,22 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int,
                     dt = due - time.perf_counter()
                     if dt > 0:
                         time.sleep(dt)
-                ids[i] = srv.submit(m, -1, P.RCM_FLAG_STABLE)
+                ids[i] = srv.submit(m, -1, P.RCM_FLAG_STABLE, borrow=borrow)
+            submitted = time.perf_counter() - t0
             th.join()
             wall = time.perf_counter() - t0
             st = srv.stats()
-        return wall, lat, dev, que, st, res
+        return wall, lat, dev, que, st, res, submitted
 
-    wall, lat, dev, que, st, res = serve(None)
+    # the requests' arrays outlive their requests here, so the server reads them in place (RCM_FLAG_BORROW), as
+    # the one-at-a-time loop above uploads from them; the copying submit is measured beside it
+    wall_copy, *_, sub_copy = serve(None, borrow=False)
+    out["capacity_copying_submit_per_s"] = len(reqs) / wall_copy
+    out["capacity_copying_submit_s"] = sub_copy
+    wall, lat, dev, que, st, res, sub = serve(None)
     cap = len(reqs) / wall
     out["capacity_per_s"] = cap
+    out["capacity_submit_s"] = sub
     out["capacity_p50_ms"] = float(np.percentile(lat, 50))
     out["capacity_p99_ms"] = float(np.percentile(lat, 99))
     out["capacity_groups"] = int(st.loop_iterations)
@@ -257,13 +264,14 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int,
         "search_and_order": st.group_order_ms / g, "bandwidth_d2h": st.group_readback_ms / g,
         "deliver": st.group_deliver_ms / g, "launches": st.group_launches / g}
     out["speedup_vs_one_at_a_time"] = cap / out["one_at_a_time_per_s"]
-    print(f"[bench] serving capacity {cap:.1f} requests/s ({out['speedup_vs_one_at_a_time']:.1f}x one at a time)",
-          file=sys.stderr, flush=True)
+    print(f"[bench] serving capacity {cap:.1f} requests/s ({out['speedup_vs_one_at_a_time']:.1f}x one at a time; "
+          f"submitting took {sub * 1e3:.0f} ms); with a copying submit {out['capacity_copying_submit_per_s']:.1f}/s "
+          f"(submitting took {sub_copy * 1e3:.0f} ms)", file=sys.stderr, flush=True)
     out["verified"] = verify_all(reqs, [r[0] for r in res], [r[1] for r in res])
     for f in fracs:
         rate = f * cap
         gaps = rng.exponential(1.0 / rate, size=len(reqs))
-        wall, lat, dev, que, st, _ = serve(gaps)
+        wall, lat, dev, que, st, _, _ = serve(gaps)
         out["runs"].append({
             "fraction_of_capacity": f, "arrival_rate_per_s": rate, "throughput_per_s": len(reqs) / wall,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

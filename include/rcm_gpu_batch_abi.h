@@ -36,7 +36,10 @@ extern "C" {
 enum rcm_flags {
   RCM_FLAG_REVERSE = 1u << 0,   /* return RCM (reversed CM) instead of CM */
   RCM_FLAG_STABLE = 1u << 1,    /* stable degree sort (always on) */
-  RCM_FLAG_BANDWIDTH = 1u << 2  /* also compute the bandwidth of the reordered matrix */
+  RCM_FLAG_BANDWIDTH = 1u << 2, /* also compute the bandwidth of the reordered matrix */
+  RCM_FLAG_BORROW = 1u << 3     /* rcm_server_submit only: read the caller's CSR in place (no copy at submit);
+                                   the buffers must stay valid and unchanged until rcm_server_wait has returned
+                                   the request's result, or rcm_server_destroy has returned */
 };
 
 enum rcm_status {
@@ -294,7 +297,8 @@ typedef struct rcm_server_stats {
 } rcm_server_stats;
 
 int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out);
-/* Queue a request (the CSR is copied; the caller's buffers may be reused at once). start_node < 0 picks the
+/* Queue a request (the CSR is copied; the caller's buffers may be reused at once; with RCM_FLAG_BORROW it is read
+ * in place when the request's group is formed, so they must outlive the request). start_node < 0 picks the
  * pseudo-peripheral start. Returns RCM_ENOMEM if the matrix can never fit a lane's arena. */
 int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
                       int32_t start_node, uint32_t flags, int64_t* id_out);

@@ -35,6 +35,7 @@ from ._native import (
 RCM_FLAG_REVERSE = 1
 RCM_FLAG_STABLE = 2
 RCM_FLAG_BANDWIDTH = 4
+RCM_FLAG_BORROW = 8
 
 RCM_OK = 0
 RCM_EINVAL = -1
@@ -792,12 +793,20 @@ class RcmServer:
         _check(_native.lib().rcm_server_create(device, C.byref(cfg), C.byref(h)), "rcm_server_create")
         self._h = h
         self._n = {}
+        self._keep = {}  # borrowed arrays of requests not yet waited for
 
-    def submit(self, m: CSR, start: int = -1, flags: int = RCM_FLAG_STABLE) -> int:
+    def submit(self, m: CSR, start: int = -1, flags: int = RCM_FLAG_STABLE, borrow: bool = False) -> int:
+        """Queue m. With borrow the server reads m's arrays in place (RCM_FLAG_BORROW): no copy at submit; this
+        object keeps them alive until the request's wait returns, and they must not be changed before then."""
         jid = C.c_int64()
-        _check(_native.lib().rcm_server_submit(self._h, m.rows, m.nnz, _p32(m.row_offsets), _p32(m.col_ids),
-                                               int(start), flags, C.byref(jid)), "rcm_server_submit")
+        rp, ci = m.row_offsets, m.col_ids
+        if borrow:
+            flags |= RCM_FLAG_BORROW
+        _check(_native.lib().rcm_server_submit(self._h, m.rows, m.nnz, _p32(rp), _p32(ci), int(start), flags,
+                                               C.byref(jid)), "rcm_server_submit")
         self._n[jid.value] = m.rows
+        if flags & RCM_FLAG_BORROW:
+            self._keep[jid.value] = (rp, ci)
         return jid.value
 
     def wait(self, jid: int, timeout_ms: int = -1):
@@ -809,6 +818,7 @@ class RcmServer:
         if rc == RCM_EAGAIN:
             raise RcmError(rc, "rcm_server_wait")
         del self._n[jid]
+        self._keep.pop(jid, None)
         return rc, (perm if rc == RCM_OK else None), st
 
     def stats(self) -> ServerStatsT:
@@ -820,6 +830,7 @@ class RcmServer:
         if self._h:
             _native.lib().rcm_server_destroy(self._h)
             self._h = None
+            self._keep.clear()
 
     def __enter__(self):
         return self

@@ -678,6 +678,8 @@ struct ServerJob {
   int32_t n = 0;
   int64_t nnz = 0;
   std::vector<int32_t> rp, ci;
+  const int32_t* src_rp = nullptr;  // the CSR the group reads: the copies above, or the caller's (RCM_FLAG_BORROW)
+  const int32_t* src_ci = nullptr;
   int32_t start_req = -1;
   uint32_t flags = 0;
   size_t bytes = 0;  // arena bytes of this matrix
@@ -768,8 +770,8 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
   // these stay within the 16 cores a GPU's host share has)
   auto copy_range = [&](int32_t k0, int32_t k1) {
     for (int32_t k = k0; k < k1; ++k) {
-      std::copy(grp[k]->rp.begin(), grp[k]->rp.end(), rp_cat + rp_off[k]);
-      std::copy(grp[k]->ci.begin(), grp[k]->ci.end(), ci_cat + ci_off[k]);
+      std::copy(grp[k]->src_rp, grp[k]->src_rp + n[k] + 1, rp_cat + rp_off[k]);
+      std::copy(grp[k]->src_ci, grp[k]->src_ci + nnz[k], ci_cat + ci_off[k]);
     }
   };
   constexpr int kCopyThreads = 4;
@@ -914,6 +916,7 @@ void lane_loop(rcm_server* s, int lane) {
         jp->rp.shrink_to_fit();
         jp->ci.clear();
         jp->ci.shrink_to_fit();
+        jp->src_rp = jp->src_ci = nullptr;
         if (jp->status == RCM_OK) ++s->st.completed;
         else ++s->st.failed;
       }
@@ -2020,8 +2023,15 @@ int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_
   auto jp = std::make_shared<ServerJob>();
   jp->n = n;
   jp->nnz = nnz;
-  jp->rp.assign(row_ptr, row_ptr + (size_t)n + 1);
-  if (nnz > 0) jp->ci.assign(col_idx, col_idx + nnz);
+  if (flags & RCM_FLAG_BORROW) {  // read in place when the group is formed
+    jp->src_rp = row_ptr;
+    jp->src_ci = col_idx;
+  } else {
+    jp->rp.assign(row_ptr, row_ptr + (size_t)n + 1);
+    if (nnz > 0) jp->ci.assign(col_idx, col_idx + nnz);
+    jp->src_rp = jp->rp.data();
+    jp->src_ci = jp->ci.data();
+  }
   jp->start_req = start_node;
   jp->flags = flags;
   jp->bytes = bytes;

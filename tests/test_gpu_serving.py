@@ -101,3 +101,27 @@ def test_server_admission_by_arena_memory():
     assert st.arena_peak_bytes <= arena
     for m, (rc, perm, _) in zip(mats, res):
         assert rc == 0 and np.array_equal(perm, O.cm(m.row_offsets, m.col_ids, 0))
+
+
+def test_server_borrowed_requests():
+    """RCM_FLAG_BORROW: the server reads the caller's arrays in place. Requests whose only reference is the
+    wrapper's (the arrays below are dropped at once) match the oracle, mixed with copied ones and a malformed
+    borrowed one."""
+    mats = _mix(12, 24)
+    _say("server borrowed requests")
+    with P.RcmServer(slots=5, lanes=2) as srv:
+        ids = []
+        for k, m in enumerate(mats):
+            m2 = P.CSR(m.row_offsets.copy(), m.col_ids.copy())  # the server's reference is the only one left
+            ids.append(srv.submit(m2, -1, P.RCM_FLAG_BANDWIDTH, borrow=k % 3 != 1))
+            del m2
+        bad = srv.submit(P.CSR(np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32)), 0, borrow=True)
+        out = [srv.wait(i, 60000) for i in ids]
+        assert srv.wait(bad, 30000)[0] == P.RCM_EINVAL
+        assert not srv._keep
+    for m, (rc, perm, js) in zip(mats, out):
+        assert rc == 0
+        _, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+        want = O.cm(m.row_offsets, m.col_ids, s)
+        assert js.start_node == s and np.array_equal(perm, want)
+        assert js.bandwidth == O.bandwidth(m.row_offsets, m.col_ids, want)
