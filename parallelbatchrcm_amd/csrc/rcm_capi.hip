@@ -766,32 +766,33 @@ void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>&
     rp_cat = rv.data();
     ci_cat = cv.data();
   }
-  // the concatenation: large groups are split over kCopyThreads host threads by bytes (the lanes' threads and
-  // these stay within the 16 cores a GPU's host share has)
-  auto copy_range = [&](int32_t k0, int32_t k1) {
-    for (int32_t k = k0; k < k1; ++k) {
-      std::copy(grp[k]->src_rp, grp[k]->src_rp + n[k] + 1, rp_cat + rp_off[k]);
-      std::copy(grp[k]->src_ci, grp[k]->src_ci + nnz[k], ci_cat + ci_off[k]);
+  // the concatenation: the group's 2C segments (each request's row offsets and columns) laid end to end, and
+  // words [w0, w1) of that sequence copied by one thread; a large group is split over kCopyThreads host threads
+  // by words, so one large request is split too (the lanes' threads and these stay within the 16 cores a GPU's
+  // host share has)
+  auto copy_words = [&](int64_t w0, int64_t w1) {
+    int64_t base = 0;
+    for (int32_t k = 0; k < C && base < w1; ++k) {
+      const int32_t* src[2] = {grp[k]->src_rp, grp[k]->src_ci};
+      int32_t* dst[2] = {rp_cat + rp_off[k], ci_cat + ci_off[k]};
+      const int64_t len[2] = {(int64_t)n[k] + 1, nnz[k]};
+      for (int h = 0; h < 2; ++h) {
+        const int64_t a = std::max(w0, base), e = std::min(w1, base + len[h]);
+        if (a < e) std::copy(src[h] + (a - base), src[h] + (e - base), dst[h] + (a - base));
+        base += len[h];
+      }
     }
   };
   constexpr int kCopyThreads = 4;
   const int64_t words = rpn + cin;
-  if (words >= ((int64_t)1 << 18) && C > 1) {
+  if (words >= ((int64_t)1 << 18)) {
     std::vector<std::thread> th;
-    int32_t k0 = 0;
-    for (int p = 0; p < kCopyThreads && k0 < C; ++p) {
-      const int64_t goal = words * (p + 1) / kCopyThreads;  // split by the running word count
-      int32_t k1 = k0;
-      while (k1 < C && (p == kCopyThreads - 1 || rp_off[k1] + ci_off[k1] < goal)) ++k1;
-      if (k1 == k0) k1 = k0 + 1;
-      if (p == kCopyThreads - 1) k1 = C;
-      th.emplace_back(copy_range, k0, k1);
-      k0 = k1;
-    }
-    if (k0 < C) copy_range(k0, C);
+    for (int p = 1; p < kCopyThreads; ++p)
+      th.emplace_back(copy_words, words * p / kCopyThreads, words * (p + 1) / kCopyThreads);
+    copy_words(0, words / kCopyThreads);
     for (auto& x : th) x.join();
   } else {
-    copy_range(0, C);
+    copy_words(0, words);
   }
   copy_ms = ms_since(t);
   t = clk::now();
@@ -877,10 +878,14 @@ void lane_loop(rcm_server* s, int lane) {
       const size_t cap = std::min<size_t>((size_t)s->cfg.slots, std::max<size_t>(1, share));
       GroupSize gs;
       size_t skipped = 0;
+      // with pinned staging, a group stays within it (a request larger than it goes alone, through pageable
+      // staging): its upload and download then run as direct DMA
+      const bool pinned = s->pinned[(size_t)lane] != nullptr;
       for (size_t i = 0; i < s->pending.size() && skipped < 8 && grp.size() < cap;) {
         const auto& jp = s->pending[i];
         const GroupSize next = gs.plus(jp->n, jp->nnz);
-        if ((jp->n >= kServeLargeN) == large && next.bytes() <= s->lane_bytes) {
+        const bool fits_pin = !pinned || grp.empty() || next.rp + std::max<size_t>(next.ci, 1) <= kServePinnedWords;
+        if ((jp->n >= kServeLargeN) == large && next.bytes() <= s->lane_bytes && fits_pin) {
           gs = next;
           grp.push_back(jp);
           s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
