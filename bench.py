@@ -255,6 +255,7 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int,
     cap = len(reqs) / wall
     out["capacity_per_s"] = cap
     out["capacity_submit_s"] = sub
+    out["pinned_lanes"] = int(st.pinned_lanes)
     out["capacity_p50_ms"] = float(np.percentile(lat, 50))
     out["capacity_p99_ms"] = float(np.percentile(lat, 99))
     out["capacity_groups"] = int(st.loop_iterations)
@@ -266,7 +267,8 @@ def bench_serving(requests: int, fracs, seed: int, lanes: int, cold_sample: int,
     out["speedup_vs_one_at_a_time"] = cap / out["one_at_a_time_per_s"]
     print(f"[bench] serving capacity {cap:.1f} requests/s ({out['speedup_vs_one_at_a_time']:.1f}x one at a time; "
           f"submitting took {sub * 1e3:.0f} ms); with a copying submit {out['capacity_copying_submit_per_s']:.1f}/s "
-          f"(submitting took {sub_copy * 1e3:.0f} ms)", file=sys.stderr, flush=True)
+          f"(submitting took {sub_copy * 1e3:.0f} ms); lanes with pinned staging {st.pinned_lanes} of {lanes}",
+          file=sys.stderr, flush=True)
     out["verified"] = verify_all(reqs, [r[0] for r in res], [r[1] for r in res])
     for f in fracs:
         rate = f * cap

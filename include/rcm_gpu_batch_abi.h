@@ -294,6 +294,7 @@ typedef struct rcm_server_stats {
    * D2H copy of the orders; splitting the results per request. */
   double group_host_copy_ms, group_load_ms, group_order_ms, group_readback_ms, group_deliver_ms;
   int64_t group_launches;                 /* engine launches of all groups */
+  int32_t pinned_lanes;                   /* lanes holding pinned host staging (the others stage in pageable memory) */
 } rcm_server_stats;
 
 int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out);

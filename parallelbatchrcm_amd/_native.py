@@ -114,6 +114,7 @@ class ServerStatsT(C.Structure):
         ("group_readback_ms", C.c_double),
         ("group_deliver_ms", C.c_double),
         ("group_launches", C.c_int64),
+        ("pinned_lanes", C.c_int32),
     ]
 
 

@@ -2012,6 +2012,8 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
                       hipHostMallocDefault) != hipSuccess) {
       s->pinned[(size_t)l] = nullptr;
       (void)hipGetLastError();  // clear the failed call's error
+    } else {
+      ++s->st.pinned_lanes;
     }
   for (int l = 0; l < s->lanes; ++l) s->threads.emplace_back(lane_loop, s, l);
   *out = s;
