@@ -326,6 +326,13 @@ int rcm_gpu_batch_many(int32_t count, const int32_t* n, const int64_t* nnz, cons
  * total (in every word), and per 64-lane wave the sum, the maximum, the minimum and the inclusive scan. */
 int rcm_selftest_primitives(int device, const uint32_t* in, uint32_t* out);
 
+/* No reference counterpart. The HBM bandwidth a plain device-to-device copy reaches: `reps` copies of `bytes`
+ * (a multiple of 16, at most 64 GiB) between two fresh device buffers, after two warm-up copies, timed with HIP
+ * events; mode 0: the library's grid-stride copy kernel, 1: hipMemcpyAsync device to device. *gbs_out = bytes
+ * read + written per second / 1e9; *ms_out (nullable) = one copy's milliseconds. */
+int rcm_selftest_copy_bandwidth(int device, int64_t bytes, int32_t reps, int32_t mode, double* gbs_out,
+                                double* ms_out);
+
 /* ---- autotuner ---- */
 /* No reference counterpart. Times every candidate (engine variant x persistent workgroups, and for rcm_ctx the
  * level-synchronous variant 5; rcm_batch then also tries helper caps of 8192, 32768 and 131072 nodes) on the

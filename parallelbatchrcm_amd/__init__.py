@@ -780,6 +780,17 @@ class RcmBatch:
             pass
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, device: int = 0,
+                   runtime: bool = False) -> tuple[float, float]:
+    """(GB/s, ms per copy) of a plain device-to-device copy of nbytes (rcm_selftest_copy_bandwidth): the HBM
+    bandwidth this GPU reaches on streaming reads and writes; the library's copy kernel, or with runtime the HIP
+    runtime's own device-to-device copy."""
+    gbs, ms = C.c_double(), C.c_double()
+    _check(_native.lib().rcm_selftest_copy_bandwidth(device, int(nbytes), int(reps), 1 if runtime else 0,
+                                                     C.byref(gbs), C.byref(ms)), "rcm_selftest_copy_bandwidth")
+    return gbs.value, ms.value
+
+
 class RcmServer:
     """Continuous batching of reorder requests on one GPU (rcm_server in include/rcm_gpu_batch_abi.h): lanes of
     many-matrix contexts run side by side; a free lane takes every waiting request that fits its HBM arena and

@@ -178,6 +178,8 @@ _SIGNATURES = {
     ),
     "rcm_status_string": (C.c_char_p, [C.c_int]),
     "rcm_selftest_primitives": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "rcm_selftest_copy_bandwidth": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]),
     # rcm_workloads.h
     "rcm_csr_free": (None, [C.POINTER(CsrT)]),
     "rcm_gen_stencil2d": (C.c_int32, [C.c_int32, C.c_int32, C.POINTER(CsrT)]),

@@ -37,3 +37,17 @@ def test_primitives_match_numpy(name):
     np.testing.assert_array_equal(out[768:1024], np.repeat(w.max(1), 64).astype(np.uint32))
     np.testing.assert_array_equal(out[1024:1280], np.repeat(w.min(1), 64).astype(np.uint32))
     np.testing.assert_array_equal(out[1280:1536], np.cumsum(w, axis=1).reshape(-1).astype(np.uint32))
+
+
+def test_copy_bandwidth_is_plausible():
+    """rcm_selftest_copy_bandwidth: a plain device copy reaches a bandwidth between 5% of the MI355X HBM peak and
+    the peak itself (with 10% allowance for the clock of the event timer); malformed sizes are rejected on the
+    host."""
+    print("[case] copy bandwidth", flush=True)
+    for runtime in (False, True):
+        gbs, ms = P.copy_bandwidth(1 << 28, 3, runtime=runtime)
+        assert 400.0 < gbs < 8800.0, (runtime, gbs)
+        assert abs(2 * (1 << 28) / (ms * 1e-3) / 1e9 - gbs) < 1e-6 * gbs
+    for bad in (0, 15, (1 << 20) + 8):
+        with pytest.raises(P.RcmError):
+            P.copy_bandwidth(bad, 1)
