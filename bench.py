@@ -545,8 +545,9 @@ def main() -> int:
     if args.pmc_probe:
         return pmc_probe(args.pmc_probe, args.claim_filter, args.variant, args.workers, args.pmc_legs)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The batched form shards by matrix; see DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}). A set of "
+              "matrices would shard by matrix, one process per GPU, each running the batched form on its share.",
+              file=sys.stderr)
         return 2
 
     import numpy as np
