@@ -677,7 +677,7 @@ struct ServerJob {
   int64_t id = 0;
   int32_t n = 0;
   int64_t nnz = 0;
-  std::vector<int32_t> rp, ci;
+  std::unique_ptr<int32_t[]> rp, ci;  // the submit's copies (left uninitialised before the copy)
   const int32_t* src_rp = nullptr;  // the CSR the group reads: the copies above, or the caller's (RCM_FLAG_BORROW)
   const int32_t* src_ci = nullptr;
   int32_t start_req = -1;
@@ -917,10 +917,8 @@ void lane_loop(rcm_server* s, int lane) {
       for (auto& jp : grp) {
         jp->t_done = t;
         jp->done = true;
-        jp->rp.clear();
-        jp->rp.shrink_to_fit();
-        jp->ci.clear();
-        jp->ci.shrink_to_fit();
+        jp->rp.reset();
+        jp->ci.reset();
         jp->src_rp = jp->src_ci = nullptr;
         if (jp->status == RCM_OK) ++s->st.completed;
         else ++s->st.failed;
@@ -2092,10 +2090,30 @@ int rcm_server_submit(rcm_server* s, int32_t n, int64_t nnz, const int32_t* row_
     jp->src_rp = row_ptr;
     jp->src_ci = col_idx;
   } else {
-    jp->rp.assign(row_ptr, row_ptr + (size_t)n + 1);
-    if (nnz > 0) jp->ci.assign(col_idx, col_idx + nnz);
-    jp->src_rp = jp->rp.data();
-    jp->src_ci = jp->ci.data();
+    // fresh pages are faulted in by the copy itself (no zero fill first), and a large request's copy is split
+    // over kSubmitThreads threads so that its page faults are taken in parallel
+    jp->rp.reset(new (std::nothrow) int32_t[(size_t)n + 1]);
+    if (nnz > 0) jp->ci.reset(new (std::nothrow) int32_t[(size_t)nnz]);
+    if (!jp->rp || (nnz > 0 && !jp->ci)) return RCM_ENOMEM;
+    constexpr int kSubmitThreads = 4;
+    const int64_t words = (int64_t)n + 1 + nnz;
+    auto copy_words = [&](int64_t w0, int64_t w1) {  // words [w0, w1) of row_ptr followed by col_idx
+      const int64_t nr = (int64_t)n + 1;
+      if (w0 < nr) std::copy(row_ptr + w0, row_ptr + std::min(w1, nr), jp->rp.get() + w0);
+      const int64_t c0 = std::max(w0, nr) - nr;
+      if (w1 > nr) std::copy(col_idx + c0, col_idx + (w1 - nr), jp->ci.get() + c0);
+    };
+    if (words >= ((int64_t)1 << 22)) {
+      std::vector<std::thread> th;
+      for (int p = 1; p < kSubmitThreads; ++p)
+        th.emplace_back(copy_words, words * p / kSubmitThreads, words * (p + 1) / kSubmitThreads);
+      copy_words(0, words / kSubmitThreads);
+      for (auto& x : th) x.join();
+    } else {
+      copy_words(0, words);
+    }
+    jp->src_rp = jp->rp.get();
+    jp->src_ci = jp->ci.get();
   }
   jp->start_req = start_node;
   jp->flags = flags;
