@@ -151,6 +151,114 @@ void dev_free(T*& p) {
   p = nullptr;
 }
 
+// Uploads of a CSR from pageable host memory. kUpThreads host threads each take an equal share of the bytes and
+// copy it in kUpChunk pieces into two pinned buffers of their own, alternating, with the DMA of one piece (on the
+// thread's own stream) overlapping the copy of the next. Measured on MI355X for 2 GiB: 42-45 GB/s, against 28 GB/s
+// for one hipMemcpyAsync from pageable memory and 35-37 GB/s for four of them from four threads. One uploader per
+// device, made on the first upload of at least kUpMin bytes and kept until the process ends (2 x kUpThreads x
+// kUpChunk = 128 MiB of pinned memory); uploads on one device take turns. Without it (the pinned allocation
+// failed) the bytes go by hipMemcpyAsync.
+constexpr int kUpThreads = 4;
+constexpr size_t kUpChunk = (size_t)16 << 20;
+constexpr size_t kUpMin = (size_t)32 << 20;
+constexpr int kUpDevices = 64;
+
+struct UpSeg {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+struct Uploader {
+  std::mutex mu;
+  int state = 0;  // 0 not made yet, 1 ready, -1 unavailable
+  char* pin[kUpThreads][2] = {};
+  hipEvent_t ev[kUpThreads][2] = {};
+  hipStream_t st[kUpThreads] = {};
+
+  bool ready() {  // with the device current and mu held
+    if (state) return state > 0;
+    bool ok = true;
+    for (int p = 0; p < kUpThreads && ok; ++p) {
+      ok = hipStreamCreateWithFlags(&st[p], hipStreamNonBlocking) == hipSuccess;
+      for (int k = 0; k < 2 && ok; ++k)
+        ok = hipHostMalloc(reinterpret_cast<void**>(&pin[p][k]), kUpChunk, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&ev[p][k], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+      (void)hipGetLastError();
+      for (int p = 0; p < kUpThreads; ++p) {
+        for (int k = 0; k < 2; ++k) {
+          if (pin[p][k]) (void)hipHostFree(pin[p][k]);
+          if (ev[p][k]) (void)hipEventDestroy(ev[p][k]);
+          pin[p][k] = nullptr;
+          ev[p][k] = nullptr;
+        }
+        if (st[p]) (void)hipStreamDestroy(st[p]);
+        st[p] = nullptr;
+      }
+    }
+    state = ok ? 1 : -1;
+    return ok;
+  }
+
+  // Thread p's share [r0, r1) of the segments laid end to end.
+  hipError_t share(const UpSeg* seg, int ns, int p, size_t r0, size_t r1) {
+    hipError_t err = hipSuccess;
+    int c = 0;
+    size_t base = 0;
+    for (int s = 0; s < ns && base < r1; base += seg[s].bytes, ++s) {
+      const size_t a = std::max(r0, base), e = std::min(r1, base + seg[s].bytes);
+      for (size_t off = a; off < e && err == hipSuccess; off += kUpChunk, ++c) {
+        const size_t len = std::min(kUpChunk, e - off);
+        const int k = c & 1;
+        if (c >= 2) err = hipEventSynchronize(ev[p][k]);  // the buffer's previous piece has reached the device
+        if (err != hipSuccess) break;
+        std::memcpy(pin[p][k], static_cast<const char*>(seg[s].src) + (off - base), len);
+        err = hipMemcpyAsync(static_cast<char*>(seg[s].dst) + (off - base), pin[p][k], len, hipMemcpyHostToDevice,
+                             st[p]);
+        if (err == hipSuccess) err = hipEventRecord(ev[p][k], st[p]);
+      }
+    }
+    const hipError_t e2 = hipStreamSynchronize(st[p]);
+    return err != hipSuccess ? err : e2;
+  }
+};
+
+Uploader g_up[kUpDevices];
+
+// Copies the segments to the device on `device` (current); when it returns the bytes are on the device. Work
+// already queued on `cs` finishes first (it may read the destinations); small uploads simply queue on `cs`.
+int upload_host(int device, hipStream_t cs, const UpSeg* seg, int ns) {
+  size_t total = 0;
+  for (int s = 0; s < ns; ++s) total += seg[s].bytes;
+  Uploader* u = device >= 0 && device < kUpDevices && total >= kUpMin ? &g_up[device] : nullptr;
+  std::unique_lock<std::mutex> lk;
+  if (u) {
+    lk = std::unique_lock<std::mutex>(u->mu);
+    if (!u->ready()) u = nullptr;
+  }
+  if (!u) {
+    for (int s = 0; s < ns; ++s)
+      if (seg[s].bytes) RCM_HIP(hipMemcpyAsync(seg[s].dst, seg[s].src, seg[s].bytes, hipMemcpyHostToDevice, cs));
+    return RCM_OK;
+  }
+  RCM_HIP(hipStreamSynchronize(cs));
+  hipError_t errs[kUpThreads];
+  auto bound = [&](int p) { return p >= kUpThreads ? total : (total * (size_t)p / kUpThreads) & ~(size_t)255; };
+  std::vector<std::thread> th;
+  for (int p = 1; p < kUpThreads; ++p)
+    th.emplace_back([&, p] {
+      (void)hipSetDevice(device);
+      errs[p] = u->share(seg, ns, p, bound(p), bound(p + 1));
+    });
+  errs[0] = u->share(seg, ns, 0, bound(0), bound(1));
+  for (auto& x : th) x.join();
+  for (hipError_t e : errs)
+    if (e != hipSuccess) return RCM_EDEVICE;
+  return RCM_OK;
+}
+
 }  // namespace
 
 struct rcm_ctx {
@@ -1248,10 +1356,10 @@ int rcm_ctx_load(rcm_ctx* c, int32_t n, int64_t nnz, const int32_t* row_ptr, con
     c->cap_ci = std::max<int64_t>(nnz, 1);
   }
   c->own_csr = true;
-  RCM_HIP(hipMemcpyAsync(c->rp, row_ptr, ((size_t)n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  if (nnz > 0)
-    RCM_HIP(hipMemcpyAsync(c->ci, col_idx, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  const UpSeg seg[2] = {{c->rp, row_ptr, ((size_t)n + 1) * sizeof(int32_t)},
+                        {c->ci, col_idx, (size_t)nnz * sizeof(int32_t)}};
   c->n = 0;
+  if (const int up = upload_host(c->device, c->stream, seg, 2)) return up;
   const int rc = load_common(c, n, nnz);
   c->timing.load_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1635,9 +1743,15 @@ int rcm_batch_load(rcm_batch* b, int32_t count, const int32_t* n, const int64_t*
   b->order.resize(C);
   for (int32_t k = 0; k < count; ++k) b->order[k] = k;
   std::stable_sort(b->order.begin(), b->order.end(), [&](int32_t x, int32_t y) { return nnz[x] > nnz[y]; });
-  RCM_HIP(hipMemcpyAsync(b->rp, row_ptr_cat, (size_t)rp_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-  if (ci_end > 0)
-    RCM_HIP(hipMemcpyAsync(b->ci, col_idx_cat, (size_t)ci_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  if (b->arena_fixed) {  // a serving lane: its groups are staged in pinned memory already
+    RCM_HIP(hipMemcpyAsync(b->rp, row_ptr_cat, (size_t)rp_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    if (ci_end > 0)
+      RCM_HIP(hipMemcpyAsync(b->ci, col_idx_cat, (size_t)ci_end * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+  } else {
+    const UpSeg seg[2] = {{b->rp, row_ptr_cat, (size_t)rp_end * sizeof(int32_t)},
+                          {b->ci, col_idx_cat, (size_t)ci_end * sizeof(int32_t)}};
+    if (const int up = upload_host(b->device, b->stream, seg, 2)) return up;
+  }
   RCM_HIP(hipMemcpyAsync(b->jobs, jobs.data(), C * sizeof(JobDesc), hipMemcpyHostToDevice, b->stream));
   b->jobs_host = jobs;
   RCM_HIP(hipMemcpyAsync(b->d_order, b->order.data(), C * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
