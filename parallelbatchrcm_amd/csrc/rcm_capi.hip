@@ -249,8 +249,8 @@ int upload_host(int device, hipStream_t cs, const UpSeg* seg, int ns) {
   std::vector<std::thread> th;
   for (int p = 1; p < kUpThreads; ++p)
     th.emplace_back([&, p] {
-      (void)hipSetDevice(device);
-      errs[p] = u->share(seg, ns, p, bound(p), bound(p + 1));
+      errs[p] = hipSetDevice(device);  // a new thread starts on device 0
+      if (errs[p] == hipSuccess) errs[p] = u->share(seg, ns, p, bound(p), bound(p + 1));
     });
   errs[0] = u->share(seg, ns, 0, bound(0), bound(1));
   for (auto& x : th) x.join();
