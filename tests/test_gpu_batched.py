@@ -285,3 +285,37 @@ def test_batch_helper_cap_keeps_results():
         want = O.cm(bs.matrix(k).row_offsets, bs.matrix(k).col_ids, int(starts[k]))
         for h, perms in got.items():
             assert np.array_equal(perms[k], want), (k, h)
+
+
+def test_large_upload_through_the_pinned_pipeline():
+    """A set whose CSRs total more than 32 MiB is uploaded by 4 host threads through pinned double buffers (pieces
+    of 16 MiB that straddle the two arrays and the matrices); then a single matrix of that size through rcm_ctx. Every
+    order and bandwidth against the oracle, from the oracle's own start."""
+    mats = [P.workloads.config(2, scale=80, seed=41), P.workloads.config(3, scale=300000, seed=42),
+            P.workloads.config(2, scale=72, seed=43)]
+    total = sum(4 * (m.rows + 1) + 4 * m.nnz for m in mats)
+    assert total > 32 << 20, total
+    want = []
+    for m in mats:
+        rc, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
+        assert rc == 0
+        want.append((s, O.cm(m.row_offsets, m.col_ids, s)))
+    bs = P.BatchSet.from_matrices(mats)
+    _say("large upload batch", len(bs), total)
+    with P.RcmBatch(timeout_ms=30000) as b:
+        b.load(bs)
+        starts, _, st = b.order()
+        assert (st == 0).all()
+        perms = b.perms()
+        bw = b.bandwidth()
+    for k, m in enumerate(mats):
+        assert starts[k] == want[k][0] and np.array_equal(perms[k], want[k][1]), k
+        assert bw[k] == O.bandwidth(m.row_offsets, m.col_ids, want[k][1]), k
+    big = P.workloads.config(2, scale=112, seed=44)  # 4 * (n + 1) + 4 * nnz > 32 MiB alone
+    assert 4 * (big.rows + 1) + 4 * big.nnz > 32 << 20
+    _say("large upload ctx")
+    with P.RcmContext(timeout_ms=30000) as c:
+        c.load(big)
+        s, _ = c.order()
+        assert np.array_equal(c.perm(), O.cm(big.row_offsets, big.col_ids, s))
+        assert s == O.peripheral(big.row_offsets, big.col_ids)[1]
