@@ -31,7 +31,7 @@ def lib(engine: str = "block"):
     return _libs[engine]
 
 
-def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=20.0, engine="block",
+def run(rp, ci, start, workers=1, slots=None, sort_mode=0, bfs=False, timeout_s=60.0, engine="block",
         claim_filter=0, levels=False):
     """Returns (perm, level, info) with info = [error, placed, batches, diag...]. levels: a CM run that also
     records BFS levels (block engine)."""
