@@ -125,3 +125,34 @@ def test_server_borrowed_requests():
         want = O.cm(m.row_offsets, m.col_ids, s)
         assert js.start_node == s and np.array_equal(perm, want)
         assert js.bandwidth == O.bandwidth(m.row_offsets, m.col_ids, want)
+
+
+def test_server_wait_limits_unknown_ids_and_shutdown_with_borrowed_requests():
+    """rcm_server_wait: RCM_EAGAIN when the request has not finished within the limit (it stays queued and is
+    waited for again), RCM_ENOTFOUND for an id the server does not hold (or held and released). Closing a server
+    with borrowed requests still queued returns once the admitted groups finish; a new server then works."""
+    big = P.connect_components(P.workloads.rmat(17, seed=5))[0]
+    _say("server wait limits")
+    with P.RcmServer(slots=2, lanes=1) as srv:
+        jid = srv.submit(big, -1, P.RCM_FLAG_STABLE, borrow=True)
+        try:
+            rc, perm, _ = srv.wait(jid, 0)  # finished already only if the device was very quick
+        except P.RcmError as e:
+            assert e.status == P.RCM_EAGAIN
+            rc, perm, _ = srv.wait(jid, 60000)
+        assert rc == 0
+        _, s, _, _ = O.peripheral(big.row_offsets, big.col_ids)
+        assert np.array_equal(perm, O.cm(big.row_offsets, big.col_ids, s))
+        lib = P._native.lib()
+        assert lib.rcm_server_wait(srv._h, jid, 0, None, None) == P.RCM_ENOTFOUND  # released by the wait above
+        assert lib.rcm_server_wait(srv._h, 1 << 40, 0, None, None) == P.RCM_ENOTFOUND
+    _say("server shutdown with borrowed requests queued")
+    mats = _mix(16, 25)
+    srv = P.RcmServer(slots=2, lanes=1)
+    for m in mats:
+        srv.submit(P.CSR(m.row_offsets.copy(), m.col_ids.copy()), -1, P.RCM_FLAG_STABLE, borrow=True)
+    srv.close()  # cancels what is queued, lets the admitted group finish
+    assert not srv._keep
+    with P.RcmServer(slots=4, lanes=2) as srv2:
+        rc, perm, _ = srv2.wait(srv2.submit(mats[0], 0), 60000)
+        assert rc == 0 and np.array_equal(perm, O.cm(mats[0].row_offsets, mats[0].col_ids, 0))
