@@ -90,7 +90,8 @@ enum rcm_param {
                                     default; serving lanes)
                                 1 = 256 threads, 4096 slots, 256 parents, 4-bit digits
                                 2 = 128 threads, 1024 slots, 128 parents, 4-bit digits
-                                3 = 256 threads, 2048 slots, 256 parents, 6-bit digits
+                                3 = 256 threads, 2048 slots, 256 parents, 6-bit digits, child batches cut at a
+                                    degree quantum of S/4 = 512 slots (S/2 in the others)
                                 4 = 512 threads, 4096 slots, 512 parents, 4-bit digits (rcm_batch default)
                                 5 = level-synchronous engine (rcm_ctx only): one BFS level per step, children
                                     ordered by a device-wide radix sort with 8-bit digits; for wide, shallow

@@ -138,13 +138,14 @@ enum : int {
 constexpr int kTraceWords = 12;
 
 // Batch geometry: T threads, at most P parents and S adjacency slots per batch.
-template <int T_, int S_, int P_>
+template <int T_, int S_, int P_, int QDIV_ = 2>
 struct Geo {
   static constexpr int T = T_, S = S_, P = P_;
   static constexpr int W = T / kWave;
   static constexpr int EPT = S / T;
-  static constexpr uint32_t Q = S / 2;  // degree quantum of the child-batch cut rule
+  static constexpr uint32_t Q = S / QDIV_;  // degree quantum of the child-batch cut rule
   static_assert(S % T == 0 && P <= T && S <= 65536, "geometry");
+  static_assert(QDIV_ >= 2, "a multi-node child batch spans at most 2Q <= S slots");
 };
 
 template <class G, int DIGIT>

@@ -230,9 +230,11 @@ def test_claim_filter_modes_agree(ctx, flt):
 
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_engine_variants_match_oracle(ctx, variant):
-    """Every engine variant (block size, batch width, radix digit) gives the same order."""
+    """Every engine variant (block size, batch width, radix digit, child-batch degree quantum) gives the same
+    order; the star's centre (degree 799) and the hubs exceed variant 3's degree quantum of 512."""
     cases = [P.CSR(*G.random_connected(500, 1500, seed=4)), P.workloads.config(2, scale=24, seed=2),
-             P.CSR(*G.hub_graph(5000, 9000, seed=1)), _rmat(14, 5)]
+             P.CSR(*G.hub_graph(5000, 9000, seed=1)), _rmat(14, 5), P.workloads.config(3, scale=40000, seed=3),
+             P.CSR(*G.star(800))]
     try:
         ctx.set_param("variant", variant)
         for m in cases:
