@@ -86,9 +86,9 @@ enum rcm_param {
   RCM_PARAM_SORT = 3,        /* 0 auto, 1 LDS radix sort, 2 segment-rank sort */
   RCM_PARAM_DIGIT_BITS = 4,  /* radix digit width of the current variant (read-only: set the variant) */
   RCM_PARAM_VARIANT = 5,     /* engine variant (block size, batch width, radix digit):
-                                0 = 256 threads, 2048 adjacency slots, 256 parents, 4-bit digits (rcm_ctx
-                                    default; serving lanes)
-                                1 = 256 threads, 4096 slots, 256 parents, 4-bit digits
+                                0 = 256 threads, 2048 adjacency slots, 256 parents, 4-bit digits (serving
+                                    lanes)
+                                1 = 512 threads, 2048 slots, 256 parents, 4-bit digits (rcm_ctx default)
                                 2 = 128 threads, 1024 slots, 128 parents, 4-bit digits
                                 3 = 256 threads, 2048 slots, 256 parents, 6-bit digits, child batches cut at a
                                     degree quantum of S/4 = 512 slots (S/2 in the others)

@@ -36,6 +36,11 @@ constexpr int64_t kDefaultHelperNodes = 8192;  // the autotuner's helper caps: n
 // spreads its workgroups over them by size (ManyArgs::share_rows), so the large quantum does not starve them.
 constexpr int64_t kDefaultBatchHelperNodes = 131072;
 constexpr int64_t kDefaultBatchVariant = 4;
+// rcm_ctx default: variant 1 (512 threads over 2048 slots: each batch phase takes 4 rounds per wave, not 8). On
+// MI355X the 128^3 stencil runs in 3.93-3.98 ms against 4.27-4.33 ms with variant 0, the 1M-point geometric graph in
+// 8.42 ms against 9.7 ms, and rcm_ctx_autotune picks it for both and for R-MAT scale 20. The serving lanes keep
+// variant 0 (capacity 914 against 673-698 requests/s with variant 1 on the 400-request stream).
+constexpr int64_t kDefaultCtxVariant = 1;
 // RCM_PARAM_SPECULATE: from this start-search sweep on, the sweep runs as a CM run that records levels. On the
 // 4096-matrix mix 98% of the matrices stop at sweep 3, 1.8% at 4 and 0.4% at 2, so speculating from sweep 3
 // turns (sweeps + 1) engine passes into sweeps for almost every matrix.
@@ -64,12 +69,12 @@ constexpr int kDefaultClaimFilter = 1;  // measured on MI355X: 1.5-7% faster, 56
 constexpr int kVariants = 5;
 template <int V> struct Var;
 template <> struct Var<0> { using G = Geo<256, 2048, 256>; static constexpr int D = 4; };
-template <> struct Var<1> { using G = Geo<256, 4096, 256>; static constexpr int D = 4; };
+template <> struct Var<1> { using G = Geo<512, 2048, 256>; static constexpr int D = 4; };
 template <> struct Var<2> { using G = Geo<128, 1024, 128>; static constexpr int D = 4; };
 template <> struct Var<3> { using G = Geo<256, 2048, 256, 4>; static constexpr int D = 6; };
 template <> struct Var<4> { using G = Geo<512, 4096, 512>; static constexpr int D = 4; };
-constexpr int kVarThreads[kVariants] = {256, 256, 128, 256, 512};
-constexpr int kVarSlots[kVariants] = {2048, 4096, 1024, 2048, 4096};
+constexpr int kVarThreads[kVariants] = {256, 512, 128, 256, 512};
+constexpr int kVarSlots[kVariants] = {2048, 2048, 1024, 2048, 4096};
 constexpr int kVarDigit[kVariants] = {4, 4, 4, 6, 4};
 // Variant kLevelVariant of a single-matrix context is the level-synchronous engine (level_cm.hpp): one BFS
 // level per step, ordered by a device-wide radix sort with 8-bit digits.
@@ -272,7 +277,7 @@ struct rcm_ctx {
   int64_t timeout_ms = 20000;
   int64_t sort_mode = 0;
   int64_t claim_filter = kDefaultClaimFilter;
-  int64_t variant = 0;
+  int64_t variant = kDefaultCtxVariant;
   int64_t strict = 0;
   int64_t profile = 0;
   int64_t batch_cap = 0;     // RCM_PARAM_BATCH_CAP: batch records a run may use (0 = n + 1, all it can need)

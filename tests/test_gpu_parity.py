@@ -235,6 +235,7 @@ def test_engine_variants_match_oracle(ctx, variant):
     cases = [P.CSR(*G.random_connected(500, 1500, seed=4)), P.workloads.config(2, scale=24, seed=2),
              P.CSR(*G.hub_graph(5000, 9000, seed=1)), _rmat(14, 5), P.workloads.config(3, scale=40000, seed=3),
              P.CSR(*G.star(800))]
+    keep = ctx.get_param("variant")
     try:
         ctx.set_param("variant", variant)
         for m in cases:
@@ -245,7 +246,14 @@ def test_engine_variants_match_oracle(ctx, variant):
             ctx.run(s)
             assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
     finally:
-        ctx.set_param("variant", 0)
+        ctx.set_param("variant", keep)
+
+
+def test_default_variant_is_the_tuned_one():
+    """A fresh context runs variant 1 (512 threads over 2048 slots), the one its autotuner picks for the 128^3
+    stencil, the geometric graph and R-MAT on MI355X."""
+    with P.RcmContext() as c:
+        assert c.get_param("variant") == 1
 
 
 def test_autotune_keeps_results(ctx):
@@ -253,6 +261,7 @@ def test_autotune_keeps_results(ctx):
     rc, s, _, _ = O.peripheral(m.row_offsets, m.col_ids)
     _say("autotune ctx", m.rows)
     ctx.load(m)
+    keep = ctx.get_param("variant")
     try:
         r = ctx.autotune(s, repeats=2)
         assert 0 <= r.variant <= 5 and r.candidates >= 11
@@ -261,13 +270,14 @@ def test_autotune_keeps_results(ctx):
         ctx.run(s)
         assert np.array_equal(ctx.perm(), O.cm(m.row_offsets, m.col_ids, s))
     finally:
-        ctx.set_param("variant", 0)
+        ctx.set_param("variant", keep)
         ctx.set_param("workers", 0)
 
 
 def test_level_synchronous_variant(ctx):
     """Variant 5, the level-synchronous engine: small graphs from several starts (duplicates, unsorted rows),
     hub graphs, R-MAT, the start search, reverse and a disconnected graph."""
+    keep = ctx.get_param("variant")
     try:
         ctx.set_param("variant", 5)
         for name, (rp, ci) in SMALL.items():
@@ -294,7 +304,7 @@ def test_level_synchronous_variant(ctx):
         ctx.load(_csr(rp2, ci2))
         assert ctx.run_status(0) == P.RCM_ENOTCONNECTED
     finally:
-        ctx.set_param("variant", 0)
+        ctx.set_param("variant", keep)
 
 
 @pytest.mark.parametrize("name,args", [("directed3000", (3000, 3000, 40, 1)), ("directed2000", (2000, 1500, 24, 2))])
