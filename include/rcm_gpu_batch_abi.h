@@ -296,6 +296,8 @@ typedef struct rcm_server_stats {
   double group_host_copy_ms, group_load_ms, group_order_ms, group_readback_ms, group_deliver_ms;
   int64_t group_launches;                 /* engine launches of all groups */
   int32_t pinned_lanes;                   /* lanes holding pinned host staging (the others stage in pageable memory) */
+  int32_t solo_lanes;                     /* lanes ordering large requests (2^18 to 2^21 rows, up to 2^26 entries)
+                                             alone through a single-matrix context */
 } rcm_server_stats;
 
 int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out);

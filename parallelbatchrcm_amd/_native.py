@@ -115,6 +115,7 @@ class ServerStatsT(C.Structure):
         ("group_deliver_ms", C.c_double),
         ("group_launches", C.c_int64),
         ("pinned_lanes", C.c_int32),
+        ("solo_lanes", C.c_int32),
     ]
 
 

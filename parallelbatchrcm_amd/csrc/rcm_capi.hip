@@ -53,6 +53,12 @@ constexpr int64_t kServerHelperNodes = 2048;
 constexpr size_t kServePinnedWords = (size_t)64 << 20;  // 256 MiB per lane
 // Serving groups requests of at least this many rows only with each other.
 constexpr int32_t kServeLargeN = 1 << 18;
+// A serving lane orders a large request (kServeLargeN rows or more) alone, through a single-matrix context whose
+// buffers are reserved at creation for up to kServeSoloN rows and kServeSoloNnz entries (no allocation while other
+// lanes' kernels run): the single-matrix engine's default variant is faster on one matrix than a one-member launch
+// of the many-matrix engine.
+constexpr int32_t kServeSoloN = 1 << 21;
+constexpr int64_t kServeSoloNnz = (int64_t)1 << 26;
 
 static_assert(sizeof(rcm_tune_result) == 40, "the Python mirror (TuneResultT) assumes this layout");
 
@@ -842,12 +848,60 @@ struct rcm_server {
   // pay for fresh zero-filled pages; and the lane's pinned staging (kServePinnedWords, null if not allocated)
   std::vector<std::vector<int32_t>> stage_rp, stage_ci, stage_perm;
   std::vector<int32_t*> pinned;
+  std::vector<rcm_ctx*> solo;  // per lane, the single-matrix context for large requests (null if not made)
 };
 
 namespace {
 
 // One lane: reorder a group of requests with the lane's many-matrix context.
+bool solo_fits(const rcm_server* s, int lane, const ServerJob& j) {
+  return s->solo[(size_t)lane] && j.n >= kServeLargeN && j.n <= kServeSoloN && j.nnz <= kServeSoloNnz;
+}
+
+// One large request through the lane's single-matrix context: upload, start search and order, bandwidth, download.
+void run_solo(rcm_server* s, int lane, ServerJob& j) {
+  rcm_ctx* c = s->solo[(size_t)lane];
+  using clk = std::chrono::steady_clock;
+  const auto ms_since = [](clk::time_point t0) {
+    return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+  };
+  auto t = clk::now();
+  int rc = rcm_ctx_load(c, j.n, j.nnz, j.src_rp, j.src_ci);  // within the reserved buffers: no allocation
+  const double load_ms = ms_since(t);
+  t = clk::now();
+  int32_t start = j.start_req, iters = 0;
+  if (!rc) rc = rcm_ctx_order(c, j.start_req, RCM_FLAG_STABLE, &start, &iters);
+  const double order_ms = ms_since(t);
+  const int launches = rc ? 0 : c->timing.order_passes;
+  t = clk::now();
+  int64_t bw = -1;
+  if (!rc && (j.flags & RCM_FLAG_BANDWIDTH)) rc = rcm_ctx_bandwidth(c, &bw);
+  if (!rc) {
+    j.perm.resize((size_t)j.n);
+    rc = rcm_ctx_read_perm(c, j.perm.data());
+  }
+  const double read_ms = ms_since(t);
+  j.start_used = start;
+  j.sweeps = iters;
+  j.status = rc;
+  if (rc) {
+    j.perm.clear();
+  } else {
+    if (j.flags & RCM_FLAG_REVERSE) std::reverse(j.perm.begin(), j.perm.end());
+    if (j.flags & RCM_FLAG_BANDWIDTH) j.bw = bw;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->st.group_load_ms += load_ms;
+  s->st.group_order_ms += order_ms;
+  s->st.group_readback_ms += read_ms;
+  s->st.group_launches += launches;
+}
+
 void run_group(rcm_server* s, int lane, std::vector<std::shared_ptr<ServerJob>>& grp) {
+  if (grp.size() == 1 && solo_fits(s, lane, *grp[0])) {
+    run_solo(s, lane, *grp[0]);
+    return;
+  }
   rcm_batch* b = s->batch[(size_t)lane];
   const int32_t C = (int32_t)grp.size();
   using clk = std::chrono::steady_clock;
@@ -990,7 +1044,10 @@ void lane_loop(rcm_server* s, int lane) {
       // burst spreads over every lane
       const bool large = s->pending.front()->n >= kServeLargeN;
       const size_t share = (s->pending.size() + (size_t)s->lanes - 1) / (size_t)s->lanes;
-      const size_t cap = std::min<size_t>((size_t)s->cfg.slots, std::max<size_t>(1, share));
+      // a large request that fits the lane's single-matrix context is ordered alone through it
+      const size_t cap = solo_fits(s, lane, *s->pending.front())
+                             ? 1
+                             : std::min<size_t>((size_t)s->cfg.slots, std::max<size_t>(1, share));
       GroupSize gs;
       size_t skipped = 0;
       // with pinned staging, a group stays within it (a request larger than it goes alone, through pageable
@@ -2192,6 +2249,23 @@ int rcm_server_create(int device, const rcm_server_config* cfg, rcm_server** out
     } else {
       ++s->st.pinned_lanes;
     }
+  s->solo.assign((size_t)s->lanes, nullptr);
+  for (int l = 0; l < s->lanes; ++l) {  // optional: without it a lane orders large requests as one-member groups
+    rcm_ctx* sc = nullptr;
+    if (rcm_ctx_create(device, &sc) != RCM_OK) continue;
+    rcm_ctx_set_param(sc, RCM_PARAM_TIMEOUT_MS, s->cfg.job_timeout_ms);
+    if (dev_alloc(&sc->rp, (size_t)kServeSoloN + 1) == hipSuccess) sc->cap_rp = (int64_t)kServeSoloN + 1;
+    if (dev_alloc(&sc->ci, (size_t)kServeSoloNnz) == hipSuccess) sc->cap_ci = kServeSoloNnz;
+    sc->own_csr = true;
+    if (sc->cap_rp < (int64_t)kServeSoloN + 1 || sc->cap_ci < kServeSoloNnz ||
+        ensure_state(sc, kServeSoloN) != RCM_OK) {
+      (void)hipGetLastError();
+      rcm_ctx_destroy(sc);
+      continue;
+    }
+    s->solo[(size_t)l] = sc;
+    ++s->st.solo_lanes;
+  }
   for (int l = 0; l < s->lanes; ++l) s->threads.emplace_back(lane_loop, s, l);
   *out = s;
   return RCM_OK;
@@ -2310,6 +2384,8 @@ int rcm_server_destroy(rcm_server* s) {
     if (t.joinable()) t.join();  // admitted groups finish first
   s->cv_done.notify_all();
   for (rcm_batch* b : s->batch) rcm_batch_destroy(b);
+  for (rcm_ctx* sc : s->solo)
+    if (sc) rcm_ctx_destroy(sc);
   for (int32_t* p : s->pinned)
     if (p) (void)hipHostFree(p);
   delete s;

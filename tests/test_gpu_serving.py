@@ -156,3 +156,35 @@ def test_server_wait_limits_unknown_ids_and_shutdown_with_borrowed_requests():
     with P.RcmServer(slots=4, lanes=2) as srv2:
         rc, perm, _ = srv2.wait(srv2.submit(mats[0], 0), 60000)
         assert rc == 0 and np.array_equal(perm, O.cm(mats[0].row_offsets, mats[0].col_ids, 0))
+
+
+def test_server_large_requests_go_through_the_single_matrix_context():
+    """Requests of 2^18 rows or more are ordered alone through each lane's single-matrix context: results equal the
+    oracle's (reverse and bandwidth flags included, borrowed and copied), and a disconnected or malformed large
+    request fails alone."""
+    big = P.workloads.config(2, scale=64, seed=7)  # 64^3 = 2^18 rows
+    assert big.rows == 1 << 18
+    half = P.workloads.config(2, scale=52, seed=8)  # two copies: 281,216 rows
+    n2 = half.rows
+    two = P.CSR(np.concatenate([half.row_offsets, half.row_offsets[1:] + half.nnz]).astype(np.int32),
+                np.concatenate([half.col_ids, half.col_ids + n2]).astype(np.int32))  # two components
+    pad = P.workloads.config(2, scale=66, seed=9)  # > 2^18 rows
+    bad_rp = pad.row_offsets.copy()
+    bad_rp[1000] = bad_rp[1002]  # row offsets decrease at row 1001
+    bad = P.CSR(bad_rp, pad.col_ids)
+    _say("server large requests")
+    with P.RcmServer(slots=4, lanes=2) as srv:
+        assert srv.stats().solo_lanes == 2
+        a = srv.submit(big, -1, P.RCM_FLAG_REVERSE | P.RCM_FLAG_BANDWIDTH, borrow=True)
+        assert two.rows >= 1 << 18
+        b = srv.submit(two, 0)
+        c = srv.submit(bad, 0)
+        d = srv.submit(big, 5)
+        ra, rb, rc_, rd = (srv.wait(x, 60000) for x in (a, b, c, d))
+    _, s, _, _ = O.peripheral(big.row_offsets, big.col_ids)
+    want = O.cm(big.row_offsets, big.col_ids, s)
+    assert ra[0] == 0 and ra[2].start_node == s and np.array_equal(ra[1], want[::-1])
+    assert ra[2].bandwidth == O.bandwidth(big.row_offsets, big.col_ids, want)
+    assert rb[0] == P.RCM_ENOTCONNECTED
+    assert rc_[0] == P.RCM_EINVAL
+    assert rd[0] == 0 and np.array_equal(rd[1], O.cm(big.row_offsets, big.col_ids, 5))
